@@ -1,0 +1,438 @@
+// CIC-deconvolved, log-binned power (the reference's powerspectrum(),
+// powerspectrum.c:35-110) on gfx950.
+//
+// * Bin index: the reference computes floor(binsperunit * log(|k|)) per mode
+//   (powerspectrum.c:38,66,75,84).  It depends only on the integer
+//   q = |k|^2, so the host evaluates that exact FP64 formula once per q and
+//   the kernels look the bin up (no device log, identical rounding).
+// * Deconvolution: invwindow() squared (fieldize.cpp:125-133,
+//   powerspectrum.c:68,77,86) = ((float)(wx*wy)*wz)^4 with per-axis float
+//   factors; the factor tables are built on the host with the reference's
+//   double formula and rounded to float as it does.
+// * Weights: 1 on the planes kz = 0 (DC skipped) and kz = d/2, 2 elsewhere
+//   (powerspectrum.c:61-89).
+// * Counts and effective k depend only on (d, nrbins): computed once per plan.
+// * Power: FP64 accumulation.  Each thread sums a run of modes that share a
+//   bin in registers and adds to per-workgroup LDS bins only at bin changes;
+//   per-workgroup partials are summed in a fixed order and normalised as in
+//   powerspectrum.c:102-108.
+// The fused kernel performs the last (x) FFT pass and bins the modes while
+// they are still in LDS, so the spectrum is never written back.
+#include "fft_core.h"
+#include "plan.h"
+
+#include <cmath>
+#include <vector>
+
+namespace genpk {
+
+int col_block(int prec, int64_t d, int64_t hz);
+
+namespace {
+
+constexpr int BIN_THREADS = 256;
+constexpr int LDS_LIMIT = 160 * 1024;
+template <typename T> struct BinCfg;
+template <> struct BinCfg<float> { static constexpr int ELEMS = 8192; static constexpr int MAXE = 32; };
+template <> struct BinCfg<double> { static constexpr int ELEMS = 4096; static constexpr int MAXE = 16; };
+
+__device__ __forceinline__ int64_t kval(int64_t n, int64_t d) { return n <= d / 2 ? n : n - d; }
+
+__device__ __forceinline__ double deconv_of(const float *__restrict__ wtab, int64_t kx, int64_t ky, int64_t kz) {
+  const float g = (wtab[kx] * wtab[ky]) * wtab[kz];
+  double w = (double)g * (double)g;  // invwindow()
+  return w * w;                      // its square, as applied per mode
+}
+
+// Data-independent per-bin mode count and sum of h*|k| over the half spectrum.
+__global__ void __launch_bounds__(BIN_THREADS) k_counts(int64_t d, int nrbins, const int *__restrict__ q2bin,
+                                                         uint32_t *__restrict__ part_cnt,
+                                                         double *__restrict__ part_keff) {
+  extern __shared__ unsigned char lds_raw[];
+  double *kb = reinterpret_cast<double *>(lds_raw);
+  uint32_t *cb = reinterpret_cast<uint32_t *>(kb + nrbins);
+  for (int i = threadIdx.x; i < nrbins; i += BIN_THREADS) {
+    kb[i] = 0;
+    cb[i] = 0;
+  }
+  __syncthreads();
+  const int64_t hz = d / 2 + 1;
+  const int64_t ncol = d * d;
+  for (int64_t c = (int64_t)blockIdx.x * BIN_THREADS + threadIdx.x; c < ncol; c += (int64_t)gridDim.x * BIN_THREADS) {
+    const int64_t kx = c / d, ky = c % d;
+    const int64_t a = kval(kx, d) * kval(kx, d) + kval(ky, d) * kval(ky, d);
+    int cur = -1;
+    uint32_t cc = 0;
+    double ck = 0;
+    for (int64_t kz = 0; kz < hz; kz++) {
+      const int64_t q = a + kz * kz;
+      if (q == 0) continue;
+      const int b = q2bin[q];
+      const uint32_t h = (kz == 0 || kz == d / 2) ? 1u : 2u;
+      if (b != cur) {
+        if (cur >= 0) {
+          atomicAdd(&cb[cur], cc);
+          atomicAdd(&kb[cur], ck);
+        }
+        cur = b;
+        cc = 0;
+        ck = 0;
+      }
+      cc += h;
+      ck += (double)h * sqrt((double)q);
+    }
+    if (cur >= 0) {
+      atomicAdd(&cb[cur], cc);
+      atomicAdd(&kb[cur], ck);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nrbins; i += BIN_THREADS) {
+    part_cnt[(int64_t)blockIdx.x * nrbins + i] = cb[i];
+    part_keff[(int64_t)blockIdx.x * nrbins + i] = kb[i];
+  }
+}
+
+__global__ void k_counts_reduce(int nrbins, int G, const uint32_t *__restrict__ part_cnt,
+                                const double *__restrict__ part_keff, int *__restrict__ count,
+                                double *__restrict__ keffsum) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nrbins) return;
+  uint64_t c = 0;
+  double k = 0;
+  for (int g = 0; g < G; g++) {
+    c += part_cnt[(int64_t)g * nrbins + b];
+    k += part_keff[(int64_t)g * nrbins + b];
+  }
+  count[b] = (int)c;
+  keffsum[b] = k;
+}
+
+// Accumulates one spectrum contribution run-by-run in registers.
+struct RunAcc {
+  int cur = -1;
+  double v[3] = {0, 0, 0};
+};
+
+template <int NS>
+__device__ __forceinline__ void run_flush(RunAcc &r, double *bins, int nrbins) {
+  if (r.cur >= 0) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) atomicAdd(&bins[s * nrbins + r.cur], r.v[s]);
+  }
+}
+
+template <int NS>
+__device__ __forceinline__ void run_add(RunAcc &r, int b, const double *c, double *bins, int nrbins) {
+  if (b != r.cur) {
+    run_flush<NS>(r, bins, nrbins);
+    r.cur = b;
+#pragma unroll
+    for (int s = 0; s < NS; s++) r.v[s] = 0;
+  }
+#pragma unroll
+  for (int s = 0; s < NS; s++) r.v[s] += c[s];
+}
+
+template <typename C, int NF>
+__device__ __forceinline__ void mode_terms(const C &fa, const C &fb, double h, double dc, double *c) {
+  const double ax = (double)fa.x, ay = (double)fa.y;
+  if (NF == 1) {
+    c[0] = h * (ax * ax + ay * ay) * dc;
+  } else {
+    const double bx = (double)fb.x, by = (double)fb.y;
+    c[0] = h * (ax * ax + ay * ay) * dc;
+    c[1] = h * (bx * bx + by * by) * dc;
+    c[2] = h * (ax * bx + ay * by) * dc;
+  }
+}
+
+struct BinGeom {
+  int64_t d, hz;
+  int nrbins;
+  const int *q2bin;
+  const float *wtab;
+};
+
+// Fused final FFT pass along x + deconvolved binning.  NF fields (1: auto,
+// 2: auto a, auto b and cross a.b*).  Persistent over column blocks.
+template <typename T, int NF>
+__global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__restrict__ fa,
+                                                            typename Cx<T>::C *__restrict__ fb, BinGeom g, int B,
+                                                            int nkb, RadixPlan rp,
+                                                            const typename Cx<T>::C *__restrict__ tw, int keep,
+                                                            double *__restrict__ part) {
+  using C = typename Cx<T>::C;
+  constexpr int NS = NF == 1 ? 1 : 3;
+  extern __shared__ unsigned char lds_raw[];
+  double *bins = reinterpret_cast<double *>(lds_raw);
+  C *buf = reinterpret_cast<C *>(bins + ((NS * g.nrbins + 1) & ~1));  // [NF][n][B], 16-byte aligned
+  const int n = rp.n;
+  for (int i = threadIdx.x; i < NS * g.nrbins; i += BIN_THREADS) bins[i] = 0;
+  const int64_t d = g.d, hz = g.hz;
+  const int64_t estride = d * hz;
+  const int64_t nblk = d * nkb;
+  // bin-phase mapping: thread -> column b, contiguous run of e
+  const int tpc = BIN_THREADS / B;               // threads per column (B divides 256)
+  const int my_b = threadIdx.x % B;
+  const int per = (n + tpc - 1) / tpc;
+  const int e0 = (threadIdx.x / B) * per;
+  for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int64_t y = blk / nkb;
+    const int kb = (int)(blk % nkb);
+    const int64_t kz0 = (int64_t)kb * B;
+    const int nb = (int)min((int64_t)B, hz - kz0);
+    const int64_t off = y * hz + kz0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n * B; i += BIN_THREADS) {
+      const int e = i / B, b = i % B;
+      buf[i] = b < nb ? fa[off + e * estride + b] : C{0, 0};
+      if (NF == 2) buf[n * B + i] = b < nb ? fb[off + e * estride + b] : C{0, 0};
+    }
+    __syncthreads();
+    lds_fft<-1, true, BinCfg<T>::MAXE>(buf, rp, B, B, 1, tw);
+    if (NF == 2) lds_fft<-1, true, BinCfg<T>::MAXE>(buf + n * B, rp, B, B, 1, tw);
+    if (keep) {
+      for (int i = threadIdx.x; i < n * B; i += BIN_THREADS) {
+        const int e = i / B, b = i % B;
+        if (b < nb) {
+          fa[off + e * estride + b] = buf[i];
+          if (NF == 2) fb[off + e * estride + b] = buf[n * B + i];
+        }
+      }
+    }
+    if (my_b < nb) {
+      const int64_t kz = kz0 + my_b;
+      const int64_t ky = kval(y, d);
+      const double h = (kz == 0 || kz == d / 2) ? 1.0 : 2.0;
+      const int64_t a = ky * ky + kz * kz;
+      RunAcc r;
+      const int e1 = min(n, e0 + per);
+      for (int e = e0; e < e1; e++) {
+        const int64_t kx = kval(e, d);
+        const int64_t q = a + kx * kx;
+        if (q == 0) continue;
+        const int b = g.q2bin[q];
+        const double dc = deconv_of(g.wtab, e, y, kz);
+        const C va = buf[e * B + my_b];
+        const C vb = NF == 2 ? buf[n * B + e * B + my_b] : va;
+        double c[3];
+        mode_terms<C, NF>(va, vb, h, dc, c);
+        run_add<NS>(r, b, c, bins, g.nrbins);
+      }
+      run_flush<NS>(r, bins, g.nrbins);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < NS * g.nrbins; i += BIN_THREADS) part[(int64_t)blockIdx.x * NS * g.nrbins + i] = bins[i];
+}
+
+// Binning of stored spectra [d][d][hz] (no FFT): one wave per (kx, ky) row,
+// each lane a contiguous run of kz.
+template <typename T, int NF>
+__global__ void __launch_bounds__(BIN_THREADS) k_bin_stored(const typename Cx<T>::C *__restrict__ fa,
+                                                             const typename Cx<T>::C *__restrict__ fb, BinGeom g,
+                                                             double *__restrict__ part) {
+  using C = typename Cx<T>::C;
+  constexpr int NS = NF == 1 ? 1 : 3;
+  extern __shared__ unsigned char lds_raw[];
+  double *bins = reinterpret_cast<double *>(lds_raw);
+  for (int i = threadIdx.x; i < NS * g.nrbins; i += BIN_THREADS) bins[i] = 0;
+  __syncthreads();
+  const int64_t d = g.d, hz = g.hz;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * (BIN_THREADS / 64) + (threadIdx.x >> 6);
+  const int64_t nwaves = (int64_t)gridDim.x * (BIN_THREADS / 64);
+  const int64_t per = (hz + 63) / 64;
+  for (int64_t row = wave; row < d * d; row += nwaves) {
+    const int64_t x = row / d, y = row % d;
+    const int64_t kx = kval(x, d), ky = kval(y, d);
+    const int64_t a = kx * kx + ky * ky;
+    RunAcc r;
+    const int64_t z0 = lane * per, z1 = min(hz, z0 + per);
+    for (int64_t kz = z0; kz < z1; kz++) {
+      const int64_t q = a + kz * kz;
+      if (q == 0) continue;
+      const int b = g.q2bin[q];
+      const double h = (kz == 0 || kz == d / 2) ? 1.0 : 2.0;
+      const double dc = deconv_of(g.wtab, x, y, kz);
+      const C va = fa[row * hz + kz];
+      const C vb = NF == 2 ? fb[row * hz + kz] : va;
+      double c[3];
+      mode_terms<C, NF>(va, vb, h, dc, c);
+      run_add<NS>(r, b, c, bins, g.nrbins);
+    }
+    run_flush<NS>(r, bins, g.nrbins);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < NS * g.nrbins; i += BIN_THREADS) part[(int64_t)blockIdx.x * NS * g.nrbins + i] = bins[i];
+}
+
+// Sum per-workgroup partials (fixed order) and normalise (powerspectrum.c:102-108).
+__global__ void k_bin_finish(int nrbins, int ns, int G, const double *__restrict__ part,
+                             const int *__restrict__ count, const double *__restrict__ keffsum, double m_aa,
+                             double m_bb, double m_ab, double *__restrict__ p_aa, double *__restrict__ p_bb,
+                             double *__restrict__ p_ab, int *__restrict__ count_out, double *__restrict__ keffs_out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nrbins) return;
+  double s[3] = {0, 0, 0};
+  for (int gi = 0; gi < G; gi++)
+    for (int k = 0; k < ns; k++) s[k] += part[((int64_t)gi * ns + k) * nrbins + b];
+  const int c = count[b];
+  double *outs[3] = {p_aa, p_bb, p_ab};
+  const double ms[3] = {m_aa, m_bb, m_ab};
+  for (int k = 0; k < ns; k++) {
+    if (!outs[k]) continue;
+    double v = 0;
+    if (c) {
+      v = s[k] / ms[k];
+      v /= c;
+    }
+    outs[k][b] = v;
+  }
+  if (count_out) count_out[b] = c;
+  if (keffs_out) keffs_out[b] = c ? keffsum[b] / c : 0.0;
+}
+
+int num_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    GENPK_HIP(hipGetDevice(&dev));
+    GENPK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  return cus;
+}
+
+}  // namespace
+
+// float(onedinvwindow(KVAL(k), d)) for k < d, the factor of fieldize.cpp:117-121
+static double onedinvwindow_host(int64_t kx, int64_t n) {
+  return kx ? M_PI * kx / (n * sin(M_PI * kx / (float)n)) : 1.0;
+}
+
+BinTables &bin_tables(Plan &p, int nrbins) {
+  GENPK_REQUIRE(nrbins >= 1 && nrbins <= 8192, "nrbins must be in [1, 8192]");
+  auto it = p.bins.find(nrbins);
+  if (it != p.bins.end() && it->second->counts_ready) return *it->second;
+  auto bt = std::make_unique<BinTables>();
+  bt->nrbins = nrbins;
+  const int64_t d = p.d, h = d / 2;
+  const int64_t qmax = 3 * h * h;
+  std::vector<int> tab((size_t)qmax + 1);
+  // binsperunit and the per-mode index exactly as powerspectrum.c:38,66
+  const double binsperunit = (nrbins - 1) / log(sqrt(3) * d / 2.0);
+  tab[0] = -1;
+  for (int64_t q = 1; q <= qmax; q++) {
+    const double kk = sqrt((double)q);
+    const int64_t b = (int64_t)floor(binsperunit * log(kk));
+    GENPK_REQUIRE(b >= 0 && b < nrbins, "a mode falls outside [0, nrbins): nrbins too small for dims");
+    tab[q] = (int)b;
+  }
+  bt->q2bin.ensure(tab.size() * 4);
+  GENPK_HIP(hipMemcpy(bt->q2bin.ptr, tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
+  if (!p.wtab.ptr) {
+    std::vector<float> w(d);
+    for (int64_t k = 0; k < d; k++) {
+      const int64_t kv = k <= d / 2 ? k : k - d;
+      w[k] = (float)onedinvwindow_host(kv, d);
+    }
+    p.wtab.ensure(d * 4);
+    GENPK_HIP(hipMemcpy(p.wtab.ptr, w.data(), d * 4, hipMemcpyHostToDevice));
+  }
+  // counts and keffs, once
+  const int G = std::min<int64_t>(2 * num_cus(), std::max<int64_t>(1, ceil_div(d * d, BIN_THREADS)));
+  DevBuf pc, pk;
+  pc.ensure((size_t)G * nrbins * 4);
+  pk.ensure((size_t)G * nrbins * 8);
+  bt->count.ensure((size_t)nrbins * 4);
+  bt->keffsum.ensure((size_t)nrbins * 8);
+  const size_t lds = (size_t)nrbins * 12;
+  hipLaunchKernelGGL(k_counts, dim3(G), dim3(BIN_THREADS), lds, p.stream, d, nrbins, bt->q2bin.as<int>(),
+                     pc.as<uint32_t>(), pk.as<double>());
+  GENPK_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_counts_reduce, dim3((unsigned)ceil_div(nrbins, 256)), dim3(256), 0, p.stream, nrbins, G,
+                     pc.as<uint32_t>(), pk.as<double>(), bt->count.as<int>(), bt->keffsum.as<double>());
+  GENPK_LAUNCH_CHECK();
+  GENPK_HIP(hipStreamSynchronize(p.stream));  // pc/pk are released on return
+  bt->counts_ready = true;
+  BinTables &ref = *bt;
+  p.bins[nrbins] = std::move(bt);
+  return ref;
+}
+
+// Largest column block for the fused x-pass + bin kernel, 0 if none fits.
+static int fused_block(int prec, int64_t d, int64_t hz, int nf, int ns, int nrbins) {
+  const int csize = 2 * prec;
+  const int elems = prec == 4 ? BinCfg<float>::ELEMS : BinCfg<double>::ELEMS;
+  int B = col_block(prec, d, hz);
+  while (B >= 1) {
+    const int64_t lds = (int64_t)nf * d * B * csize + (int64_t)((ns * nrbins + 1) & ~1) * 8;
+    if (lds <= LDS_LIMIT && d * B <= elems && BIN_THREADS % B == 0) return B;
+    B /= 2;
+  }
+  return 0;
+}
+
+template <typename T, int NF>
+static void launch_fused(Plan &a, Plan *b, const BinTables &bt, int B, bool keep, double *part, int G) {
+  using C = typename Cx<T>::C;
+  constexpr int NS = NF == 1 ? 1 : 3;
+  const int nkb = (int)ceil_div(a.hz, B);
+  const RadixPlan rp = make_radix_plan((int)a.d);
+  const C *tw = (a.prec == 4) ? (const C *)a.fft.tw_d_f32.ptr : (const C *)a.fft.tw_d_f64.ptr;
+  BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), a.wtab.as<float>()};
+  const size_t lds = (size_t)NF * a.d * B * sizeof(C) + (size_t)((NS * bt.nrbins + 1) & ~1) * 8;
+  hipLaunchKernelGGL((k_xpass_bin<T, NF>), dim3(G), dim3(BIN_THREADS), lds, a.stream, a.field.as<C>(),
+                     b ? b->field.as<C>() : (C *)nullptr, g, B, nkb, rp, tw, keep ? 1 : 0, part);
+  GENPK_LAUNCH_CHECK();
+}
+
+template <typename T, int NF>
+static void launch_stored(Plan &a, Plan *b, const BinTables &bt, double *part, int G) {
+  using C = typename Cx<T>::C;
+  constexpr int NS = NF == 1 ? 1 : 3;
+  BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), a.wtab.as<float>()};
+  hipLaunchKernelGGL((k_bin_stored<T, NF>), dim3(G), dim3(BIN_THREADS), (size_t)NS * bt.nrbins * 8, a.stream,
+                     a.field.as<C>(), b ? b->field.as<C>() : (const C *)nullptr, g, part);
+  GENPK_LAUNCH_CHECK();
+}
+
+void bin_spectra(Plan &a, Plan *b, int nrbins, bool fused_xpass, double *pow_aa, double *pow_bb, double *pow_ab,
+                 int *count, double *keffs, double ma, double mb) {
+  if (b) {
+    GENPK_REQUIRE(b->d == a.d && b->prec == a.prec, "cross spectrum needs plans of equal dims and precision");
+    GENPK_REQUIRE(b->stream == a.stream, "cross spectrum needs plans on the same stream");
+  }
+  const BinTables &bt = bin_tables(a, nrbins);
+  const int nf = b ? 2 : 1, ns = b ? 3 : 1;
+  const int G = std::max(1, std::min<int>(2 * num_cus(), (int)std::min<int64_t>(a.d * a.d, 1 << 20)));
+  double *part = static_cast<double *>(a.bin_part.ensure((size_t)G * ns * nrbins * 8));
+  if (fused_xpass) {
+    GENPK_REQUIRE(a.fstate == FieldState::YDone && (!b || b->fstate == FieldState::YDone),
+                  "fused binning needs fields after the z and y passes");
+    const int B = fused_block(a.prec, a.d, a.hz, nf, ns, nrbins);
+    GENPK_REQUIRE(B > 0, "fused x-pass does not fit in LDS for this size");
+    if (a.prec == 4) {
+      if (b) launch_fused<float, 2>(a, b, bt, B, false, part, G); else launch_fused<float, 1>(a, b, bt, B, false, part, G);
+    } else {
+      if (b) launch_fused<double, 2>(a, b, bt, B, false, part, G); else launch_fused<double, 1>(a, b, bt, B, false, part, G);
+    }
+    a.fstate = FieldState::None;
+    if (b) b->fstate = FieldState::None;
+  } else {
+    GENPK_REQUIRE(a.fstate == FieldState::Spectrum && (!b || b->fstate == FieldState::Spectrum),
+                  "binning needs transformed fields");
+    if (a.prec == 4) {
+      if (b) launch_stored<float, 2>(a, b, bt, part, G); else launch_stored<float, 1>(a, b, bt, part, G);
+    } else {
+      if (b) launch_stored<double, 2>(a, b, bt, part, G); else launch_stored<double, 1>(a, b, bt, part, G);
+    }
+  }
+  hipLaunchKernelGGL(k_bin_finish, dim3((unsigned)ceil_div(nrbins, 256)), dim3(256), 0, a.stream, nrbins, ns, G, part,
+                     bt.count.as<int>(), bt.keffsum.as<double>(), b ? ma * ma : ma * mb, mb * mb, ma * mb, pow_aa,
+                     b ? pow_bb : (double *)nullptr, b ? pow_ab : (double *)nullptr, count, keffs);
+  GENPK_LAUNCH_CHECK();
+}
+
+}  // namespace genpk

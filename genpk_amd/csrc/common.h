@@ -1,0 +1,51 @@
+// Shared definitions for the MI355X (gfx950) power-spectrum library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <cstdio>
+#include <string>
+#include <stdexcept>
+
+namespace genpk {
+
+// Error reporting.  Every C entry point catches genpk::Error and stores the
+// message in a thread-local string returned by genpk_last_error().
+struct Error : public std::runtime_error {
+  explicit Error(const std::string &m) : std::runtime_error(m) {}
+};
+
+void set_last_error(const std::string &msg);
+
+#define GENPK_HIP(call)                                                                 \
+  do {                                                                                  \
+    hipError_t _e = (call);                                                             \
+    if (_e != hipSuccess)                                                               \
+      throw ::genpk::Error(std::string(#call) + " failed: " + hipGetErrorString(_e) +   \
+                           " (" __FILE__ ":" + std::to_string(__LINE__) + ")");         \
+  } while (0)
+
+#define GENPK_REQUIRE(cond, msg)                    \
+  do {                                              \
+    if (!(cond)) throw ::genpk::Error(msg);         \
+  } while (0)
+
+#define GENPK_LAUNCH_CHECK() GENPK_HIP(hipGetLastError())
+
+// Fixed-point scale of the LDS deposit accumulators: CIC weights in [0, 1]
+// are accumulated as unsigned 64-bit integers with 36 fractional bits
+// (resolution 1.5e-11 of a particle's mass).  A work item holds at most
+// DEPOSIT_ITEM_MAX = 2^16 particles, so a cell sum stays at or below 2^52
+// and converts to double exactly.
+constexpr int FIXED_FRAC_BITS = 36;
+constexpr double FIXED_ONE = 68719476736.0;            // 2^36
+constexpr double FIXED_INV = 1.0 / 68719476736.0;      // 2^-36
+constexpr int DEPOSIT_ITEM_MAX = 65536;
+
+template <typename T> struct cplx_of;
+template <> struct cplx_of<float> { using type = float2; };
+template <> struct cplx_of<double> { using type = double2; };
+
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace genpk

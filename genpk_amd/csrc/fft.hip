@@ -1,0 +1,277 @@
+// 3-D real-to-complex FFT on the padded in-place layout the reference hands
+// to FFTW (gen-pk.cpp:176-193: real [d][d][2(d/2+1)] in, complex
+// [d][d][d/2+1] out, unnormalised, forward sign -1), and its unnormalised
+// inverse.  Three passes over HBM:
+//   z: r2c of each contiguous row; d reals are packed as d/2 complex, one
+//      d/2-point Stockham FFT in LDS, then the even/odd split to d/2+1 bins;
+//   y, x: length-d c2c FFTs over column blocks of B consecutive kz, loaded
+//      as B-wide rows (coalesced) into LDS, transformed in place.
+#include "fft_core.h"
+#include "plan.h"
+
+#include <cmath>
+#include <vector>
+
+namespace genpk {
+
+namespace {
+
+constexpr int FFT_THREADS = 256;
+template <typename T> struct FftCfg;
+template <> struct FftCfg<float> { static constexpr int ELEMS = 8192; static constexpr int MAXE = 32; };
+template <> struct FftCfg<double> { static constexpr int ELEMS = 4096; static constexpr int MAXE = 16; };
+
+// z pass, forward: rows of d reals -> d/2+1 complex, L rows per workgroup.
+template <typename T>
+__global__ void __launch_bounds__(FFT_THREADS) k_r2c_z(T *__restrict__ field, int64_t rowlen, int64_t nrows, int L,
+                                                      RadixPlan rp, const typename Cx<T>::C *__restrict__ twM,
+                                                      const typename Cx<T>::C *__restrict__ twd) {
+  using C = typename Cx<T>::C;
+  extern __shared__ unsigned char lds_raw[];
+  C *buf = reinterpret_cast<C *>(lds_raw);
+  const int M = rp.n;
+  const int64_t row0 = (int64_t)blockIdx.x * L;
+  const int nl = (int)min((int64_t)L, nrows - row0);
+  for (int i = threadIdx.x; i < L * M; i += blockDim.x) {
+    const int l = i / M, m = i % M;
+    C v = C{0, 0};
+    if (l < nl) {
+      const T *row = field + (row0 + l) * rowlen;
+      v = C{row[2 * m], row[2 * m + 1]};
+    }
+    buf[l * M + m] = v;
+  }
+  __syncthreads();
+  lds_fft<-1, false, FftCfg<T>::MAXE>(buf, rp, L, 1, M, twM);
+  // X_k = (Z_k + conj Z_{M-k})/2 - i/2 * w^k (Z_k - conj Z_{M-k}),  w = exp(-2 pi i / d)
+  const int nout = M + 1;
+  for (int i = threadIdx.x; i < nl * nout; i += blockDim.x) {
+    const int l = i / nout, k = i % nout;
+    const C zk = buf[l * M + (k == M ? 0 : k)];
+    const C zc = cconj(buf[l * M + (k == 0 ? 0 : M - k)]);
+    const C e = C{(zk.x + zc.x) * (T)0.5, (zk.y + zc.y) * (T)0.5};
+    const C o = C{(zk.y - zc.y) * (T)0.5, -(zk.x - zc.x) * (T)0.5};  // (zk - zc) / (2i)
+    const C x = cadd(e, cmul(o, twd[k]));
+    T *row = field + (row0 + l) * rowlen;
+    row[2 * k] = x.x;
+    row[2 * k + 1] = x.y;
+  }
+}
+
+// z pass, inverse: d/2+1 complex -> d reals (unnormalised c2r); the
+// imaginary parts of kz = 0 and kz = d/2 do not contribute.
+template <typename T>
+__global__ void __launch_bounds__(FFT_THREADS) k_c2r_z(T *__restrict__ field, int64_t rowlen, int64_t nrows, int L,
+                                                      RadixPlan rp, const typename Cx<T>::C *__restrict__ twM,
+                                                      const typename Cx<T>::C *__restrict__ twd) {
+  using C = typename Cx<T>::C;
+  extern __shared__ unsigned char lds_raw[];
+  C *buf = reinterpret_cast<C *>(lds_raw);  // [L][M+1]
+  const int M = rp.n;
+  const int W = M + 1;
+  const int64_t row0 = (int64_t)blockIdx.x * L;
+  const int nl = (int)min((int64_t)L, nrows - row0);
+  for (int i = threadIdx.x; i < L * W; i += blockDim.x) {
+    const int l = i / W, k = i % W;
+    C v = C{0, 0};
+    if (l < nl) {
+      const T *row = field + (row0 + l) * rowlen;
+      v = C{row[2 * k], (k == 0 || k == M) ? (T)0 : row[2 * k + 1]};
+    }
+    buf[l * W + k] = v;
+  }
+  __syncthreads();
+  // Z_k = (X_k + conj X_{M-k}) + i w^-k (X_k - conj X_{M-k}), k < M; staged in registers
+  constexpr int MAXE = FftCfg<T>::MAXE;
+  C z[MAXE];
+#pragma unroll
+  for (int q = 0; q < MAXE; q++) {
+    const int i = threadIdx.x + q * blockDim.x;
+    if (i < L * M) {
+      const int l = i / M, k = i % M;
+      const C xk = buf[l * W + k];
+      const C xc = cconj(buf[l * W + (M - k)]);
+      const C s = cadd(xk, xc);
+      C dd = csub(xk, xc);
+      const C t = twd[k];
+      dd = cmul(dd, C{t.x, -t.y});                 // times exp(+2 pi i k / d)
+      z[q] = C{s.x - dd.y, s.y + dd.x};            // s + i*dd
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXE; q++) {
+    const int i = threadIdx.x + q * blockDim.x;
+    if (i < L * M) {
+      const int l = i / M, k = i % M;
+      buf[l * M + k] = z[q];
+    }
+  }
+  __syncthreads();
+  lds_fft<+1, false, MAXE>(buf, rp, L, 1, M, twM);
+  for (int i = threadIdx.x; i < nl * M; i += blockDim.x) {
+    const int l = i / M, m = i % M;
+    const C v = buf[l * M + m];
+    T *row = field + (row0 + l) * rowlen;
+    row[2 * m] = v.x;
+    row[2 * m + 1] = v.y;
+    if (m == 0) {
+      row[2 * M] = 0;
+      row[2 * M + 1] = 0;
+    }
+  }
+}
+
+// c2c along y (axis = 1) or x (axis = 0) over column blocks of B kz values.
+template <typename T, int SIGN>
+__global__ void __launch_bounds__(FFT_THREADS) k_cols(typename Cx<T>::C *__restrict__ f, int64_t d, int64_t hz,
+                                                     int B, int nkb, int axis, RadixPlan rp,
+                                                     const typename Cx<T>::C *__restrict__ tw) {
+  using C = typename Cx<T>::C;
+  extern __shared__ unsigned char lds_raw[];
+  C *buf = reinterpret_cast<C *>(lds_raw);  // [n][B]
+  const int n = rp.n;
+  const int64_t outer = blockIdx.x / nkb;
+  const int kb = blockIdx.x % nkb;
+  const int64_t kz0 = (int64_t)kb * B;
+  const int nb = (int)min((int64_t)B, hz - kz0);
+  const int64_t estride = axis == 1 ? hz : d * hz;
+  C *base = f + (axis == 1 ? outer * d * hz : outer * hz) + kz0;
+  for (int i = threadIdx.x; i < n * B; i += blockDim.x) {
+    const int e = i / B, b = i % B;
+    buf[i] = b < nb ? base[e * estride + b] : C{0, 0};
+  }
+  __syncthreads();
+  lds_fft<SIGN, true, FftCfg<T>::MAXE>(buf, rp, B, B, 1, tw);
+  for (int i = threadIdx.x; i < n * B; i += blockDim.x) {
+    const int e = i / B, b = i % B;
+    if (b < nb) base[e * estride + b] = buf[i];
+  }
+}
+
+template <typename T>
+int rows_per_block(int64_t M) {
+  return (int)std::max<int64_t>(1, FftCfg<T>::ELEMS / std::max<int64_t>(M + 1, 1));
+}
+
+}  // namespace
+
+int col_block(int prec, int64_t d, int64_t hz) {
+  const int elems = prec == 4 ? FftCfg<float>::ELEMS : FftCfg<double>::ELEMS;
+  int B = 1;
+  while (B * 2 * d <= elems && B * 2 <= 64) B *= 2;
+  if (B > hz) {
+    B = 1;
+    while (B < hz) B *= 2;
+  }
+  return B;
+}
+
+void fft_tables_init(Plan &p) {
+  const int64_t d = p.d, M = d / 2;
+  std::vector<double> td(2 * d), tm(2 * std::max<int64_t>(M, 1));
+  std::vector<float> fd(2 * d), fm(2 * std::max<int64_t>(M, 1));
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  for (int64_t m = 0; m < d; m++) {
+    const long double a = two_pi * (long double)m / (long double)d;
+    td[2 * m] = (double)cosl(a);
+    td[2 * m + 1] = (double)(-sinl(a));
+    fd[2 * m] = (float)cosl(a);
+    fd[2 * m + 1] = (float)(-sinl(a));
+  }
+  for (int64_t m = 0; m < M; m++) {
+    const long double a = two_pi * (long double)m / (long double)M;
+    tm[2 * m] = (double)cosl(a);
+    tm[2 * m + 1] = (double)(-sinl(a));
+    fm[2 * m] = (float)cosl(a);
+    fm[2 * m + 1] = (float)(-sinl(a));
+  }
+  auto up = [&](DevBuf &b, const void *src, size_t bytes) {
+    b.ensure(bytes);
+    GENPK_HIP(hipMemcpy(b.ptr, src, bytes, hipMemcpyHostToDevice));
+  };
+  up(p.fft.tw_d_f64, td.data(), td.size() * 8);
+  up(p.fft.tw_d_f32, fd.data(), fd.size() * 4);
+  up(p.fft.tw_h_f64, tm.data(), tm.size() * 8);
+  up(p.fft.tw_h_f32, fm.data(), fm.size() * 4);
+}
+
+template <typename T>
+static void r2c_z_impl(Plan &p) {
+  using C = typename Cx<T>::C;
+  const int64_t M = p.d / 2;
+  const int L = rows_per_block<T>(M);
+  const int64_t nrows = p.d * p.d;
+  const RadixPlan rp = make_radix_plan((int)M);
+  const C *twM = (p.prec == 4) ? (const C *)p.fft.tw_h_f32.ptr : (const C *)p.fft.tw_h_f64.ptr;
+  const C *twd = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
+  hipLaunchKernelGGL(k_r2c_z<T>, dim3((unsigned)ceil_div(nrows, L)), dim3(FFT_THREADS), (size_t)L * M * sizeof(C),
+                     p.stream, p.field.as<T>(), p.rowlen, nrows, L, rp, twM, twd);
+  GENPK_LAUNCH_CHECK();
+}
+
+template <typename T>
+static void c2r_z_impl(Plan &p) {
+  using C = typename Cx<T>::C;
+  const int64_t M = p.d / 2;
+  const int L = rows_per_block<T>(M);
+  const int64_t nrows = p.d * p.d;
+  const RadixPlan rp = make_radix_plan((int)M);
+  const C *twM = (p.prec == 4) ? (const C *)p.fft.tw_h_f32.ptr : (const C *)p.fft.tw_h_f64.ptr;
+  const C *twd = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
+  hipLaunchKernelGGL(k_c2r_z<T>, dim3((unsigned)ceil_div(nrows, L)), dim3(FFT_THREADS),
+                     (size_t)L * (M + 1) * sizeof(C), p.stream, p.field.as<T>(), p.rowlen, nrows, L, rp, twM, twd);
+  GENPK_LAUNCH_CHECK();
+}
+
+template <typename T, int SIGN>
+static void cols_impl(Plan &p, int axis) {
+  using C = typename Cx<T>::C;
+  const int B = col_block(p.prec, p.d, p.hz);
+  const int nkb = (int)ceil_div(p.hz, B);
+  const RadixPlan rp = make_radix_plan((int)p.d);
+  const C *tw = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
+  hipLaunchKernelGGL((k_cols<T, SIGN>), dim3((unsigned)(p.d * nkb)), dim3(FFT_THREADS), (size_t)p.d * B * sizeof(C),
+                     p.stream, p.field.as<C>(), p.d, p.hz, B, nkb, axis, rp, tw);
+  GENPK_LAUNCH_CHECK();
+}
+
+void r2c_zpass(Plan &p) {
+  if (p.prec == 4) r2c_z_impl<float>(p); else r2c_z_impl<double>(p);
+}
+void c2r_zpass(Plan &p) {
+  if (p.prec == 4) c2r_z_impl<float>(p); else c2r_z_impl<double>(p);
+}
+void c2c_ypass(Plan &p, int sign) {
+  if (p.prec == 4) {
+    if (sign < 0) cols_impl<float, -1>(p, 1); else cols_impl<float, 1>(p, 1);
+  } else {
+    if (sign < 0) cols_impl<double, -1>(p, 1); else cols_impl<double, 1>(p, 1);
+  }
+}
+void c2c_xpass(Plan &p, int sign) {
+  if (p.prec == 4) {
+    if (sign < 0) cols_impl<float, -1>(p, 0); else cols_impl<float, 1>(p, 0);
+  } else {
+    if (sign < 0) cols_impl<double, -1>(p, 0); else cols_impl<double, 1>(p, 0);
+  }
+}
+
+void r2c(Plan &p) {
+  tiles_to_field(p);
+  GENPK_REQUIRE(p.fstate == FieldState::Real, "r2c needs a real-space density");
+  r2c_zpass(p);
+  c2c_ypass(p, -1);
+  c2c_xpass(p, -1);
+  p.fstate = FieldState::Spectrum;
+}
+
+void c2r(Plan &p) {
+  GENPK_REQUIRE(p.fstate == FieldState::Spectrum, "c2r needs a spectrum");
+  c2c_xpass(p, +1);
+  c2c_ypass(p, +1);
+  c2r_zpass(p);
+  p.fstate = FieldState::Real;
+}
+
+}  // namespace genpk

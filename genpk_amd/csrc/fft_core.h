@@ -1,0 +1,198 @@
+// LDS-resident mixed-radix Stockham FFT building blocks for gfx950.
+//
+// A workgroup holds a batch of transforms of length n in LDS, element e of
+// transform b at buf[e*se + b*sb].  Each stage of radix R reads R values
+// per butterfly (twiddled), all threads pass a barrier, then write the
+// DFT_R outputs to their autosorted positions; the buffer is reused in
+// place.  Twiddles come from a table tw[m] = exp(-2 pi i m / n), m < n,
+// conjugated for the inverse direction (SIGN = +1).
+#pragma once
+#include "common.h"
+
+namespace genpk {
+
+template <typename T> struct Cx;
+template <> struct Cx<float> { using C = float2; };
+template <> struct Cx<double> { using C = double2; };
+
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename C> __device__ __forceinline__ C cconj(C a) { return C{a.x, -a.y}; }
+// multiply by SIGN*i
+template <int SIGN, typename C> __device__ __forceinline__ C cmul_si(C a) {
+  return SIGN < 0 ? C{a.y, -a.x} : C{-a.y, a.x};
+}
+template <typename C, typename S> __device__ __forceinline__ C cscale(C a, S s) { return C{a.x * s, a.y * s}; }
+
+// In-place DFT of length R on v[0..R), exponent sign SIGN.
+template <int R, int SIGN, typename C> struct Dft;
+
+template <int SIGN, typename C> struct Dft<2, SIGN, C> {
+  __device__ __forceinline__ static void run(C *v) {
+    const C a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <int SIGN, typename C> struct Dft<4, SIGN, C> {
+  __device__ __forceinline__ static void run(C *v) {
+    const C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    const C s13 = cadd(v[1], v[3]), d13 = cmul_si<SIGN>(csub(v[1], v[3]));
+    v[0] = cadd(s02, s13);
+    v[2] = csub(s02, s13);
+    v[1] = cadd(d02, d13);
+    v[3] = csub(d02, d13);
+  }
+};
+
+template <int SIGN, typename C> struct Dft<8, SIGN, C> {
+  __device__ __forceinline__ static void run(C *v) {
+    using S = decltype(v[0].x);
+    const S h = (S)0.70710678118654752440084436210484903928;  // sqrt(1/2)
+    // radix-2 over stride 4, then two radix-4 with the 8th-root twiddles
+    C a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      a[i] = cadd(v[i], v[i + 4]);
+      b[i] = csub(v[i], v[i + 4]);
+    }
+    // b[k] *= w8^k, w8 = exp(SIGN * 2 pi i / 8)
+    b[1] = C{(b[1].x - (S)SIGN * b[1].y) * h, ((S)SIGN * b[1].x + b[1].y) * h};
+    b[2] = cmul_si<SIGN>(b[2]);
+    b[3] = C{(-b[3].x - (S)SIGN * b[3].y) * h, ((S)SIGN * b[3].x - b[3].y) * h};
+    Dft<4, SIGN, C>::run(a);
+    Dft<4, SIGN, C>::run(b);
+    v[0] = a[0]; v[2] = a[1]; v[4] = a[2]; v[6] = a[3];
+    v[1] = b[0]; v[3] = b[1]; v[5] = b[2]; v[7] = b[3];
+  }
+};
+
+template <int SIGN, typename C> struct Dft<3, SIGN, C> {
+  __device__ __forceinline__ static void run(C *v) {
+    using S = decltype(v[0].x);
+    const S c = (S)-0.5;
+    const S s = (S)SIGN * (S)0.86602540378443864676372317075293618347;  // sin(2 pi / 3)
+    const C t = cadd(v[1], v[2]);
+    const C u = csub(v[1], v[2]);
+    const C m = C{v[0].x + c * t.x, v[0].y + c * t.y};
+    const C r = C{-s * u.y, s * u.x};  // i*s*u
+    v[0] = cadd(v[0], t);
+    v[1] = cadd(m, r);
+    v[2] = csub(m, r);
+  }
+};
+
+template <int SIGN, typename C> struct Dft<5, SIGN, C> {
+  __device__ __forceinline__ static void run(C *v) {
+    using S = decltype(v[0].x);
+    const S c1 = (S)0.30901699437494742410229341718281905886;   // cos(2 pi/5)
+    const S c2 = (S)-0.80901699437494742410229341718281905886;  // cos(4 pi/5)
+    const S s1 = (S)SIGN * (S)0.95105651629515357211643933337938214340;  // sin(2 pi/5)
+    const S s2 = (S)SIGN * (S)0.58778525229247312916870595463907276860;  // sin(4 pi/5)
+    const C t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    const C u1 = csub(v[1], v[4]), u2 = csub(v[2], v[3]);
+    const C a1 = C{v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y};
+    const C a2 = C{v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y};
+    // i*(s1 u1 + s2 u2), i*(s2 u1 - s1 u2)
+    const C b1 = C{-(s1 * u1.y + s2 * u2.y), s1 * u1.x + s2 * u2.x};
+    const C b2 = C{-(s2 * u1.y - s1 * u2.y), s2 * u1.x - s1 * u2.x};
+    v[0] = cadd(v[0], cadd(t1, t2));
+    v[1] = cadd(a1, b1);
+    v[4] = csub(a1, b1);
+    v[2] = cadd(a2, b2);
+    v[3] = csub(a2, b2);
+  }
+};
+
+// Radix sequence of a length.
+struct RadixPlan {
+  int n = 1;
+  int nst = 0;
+  int r[16] = {0};
+};
+
+inline RadixPlan make_radix_plan(int n) {
+  RadixPlan p;
+  p.n = n;
+  int m = n;
+  while (m % 8 == 0) { p.r[p.nst++] = 8; m /= 8; }
+  while (m % 4 == 0) { p.r[p.nst++] = 4; m /= 4; }
+  while (m % 2 == 0) { p.r[p.nst++] = 2; m /= 2; }
+  while (m % 3 == 0) { p.r[p.nst++] = 3; m /= 3; }
+  while (m % 5 == 0) { p.r[p.nst++] = 5; m /= 5; }
+  if (m != 1) p.nst = -1;  // unsupported prime factor
+  return p;
+}
+
+inline bool fft_length_ok(int64_t n) {
+  if (n < 1 || n > (1 << 20)) return false;
+  return make_radix_plan((int)n).nst >= 0;
+}
+
+// One Stockham stage of radix R on nbatch transforms of length n held in
+// LDS.  MAXE bounds the elements a thread owns per stage.  BFAST: the
+// batch index varies fastest across threads (interleaved layout).
+template <int R, int SIGN, bool BFAST, int MAXE, typename C>
+__device__ __forceinline__ void fft_stage(C *buf, int n, int nbatch, int se, int sb, int ns,
+                                          const C *__restrict__ tw) {
+  constexpr int MAXB = (MAXE + R - 1) / R;
+  const int nq = n / R;
+  const int nbf = nbatch * nq;
+  const int twstep = n / (ns * R);
+  C v[MAXB][R];
+#pragma unroll
+  for (int i = 0; i < MAXB; i++) {
+    const int u = threadIdx.x + i * blockDim.x;
+    if (u < nbf) {
+      const int b = BFAST ? u % nbatch : u / nq;
+      const int j = BFAST ? u / nbatch : u % nq;
+      const int k = j % ns;
+      C *src = buf + b * sb + j * se;
+      v[i][0] = src[0];
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        C t = tw[r * k * twstep];
+        if (SIGN > 0) t.y = -t.y;
+        v[i][r] = cmul(src[r * nq * se], t);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < MAXB; i++) {
+    const int u = threadIdx.x + i * blockDim.x;
+    if (u < nbf) {
+      const int b = BFAST ? u % nbatch : u / nq;
+      const int j = BFAST ? u / nbatch : u % nq;
+      const int k = j % ns;
+      Dft<R, SIGN, C>::run(v[i]);
+      C *dst = buf + b * sb + ((j / ns) * ns * R + k) * se;
+#pragma unroll
+      for (int r = 0; r < R; r++) dst[r * ns * se] = v[i][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Full transform; the caller has filled buf and synchronised.
+template <int SIGN, bool BFAST, int MAXE, typename C>
+__device__ void lds_fft(C *buf, const RadixPlan &rp, int nbatch, int se, int sb, const C *__restrict__ tw) {
+  int ns = 1;
+  for (int s = 0; s < rp.nst; s++) {
+    switch (rp.r[s]) {
+      case 8: fft_stage<8, SIGN, BFAST, MAXE>(buf, rp.n, nbatch, se, sb, ns, tw); break;
+      case 4: fft_stage<4, SIGN, BFAST, MAXE>(buf, rp.n, nbatch, se, sb, ns, tw); break;
+      case 2: fft_stage<2, SIGN, BFAST, MAXE>(buf, rp.n, nbatch, se, sb, ns, tw); break;
+      case 3: fft_stage<3, SIGN, BFAST, MAXE>(buf, rp.n, nbatch, se, sb, ns, tw); break;
+      case 5: fft_stage<5, SIGN, BFAST, MAXE>(buf, rp.n, nbatch, se, sb, ns, tw); break;
+      default: break;
+    }
+    ns *= rp.r[s];
+  }
+}
+
+}  // namespace genpk

@@ -1,0 +1,132 @@
+// Plan (context) of the power-spectrum library: geometry, device buffers and
+// cached tables for one grid size and precision.
+#pragma once
+#include "common.h"
+#include <map>
+#include <memory>
+#include <vector>
+
+namespace genpk {
+
+// Grow-only device buffer owned by a plan.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  bool owned = true;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (ptr && owned) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    owned = true;
+  }
+  void attach(void *p, size_t b) {
+    release();
+    ptr = p;
+    bytes = b;
+    owned = false;
+  }
+  void *ensure(size_t want) {
+    if (want == 0) want = 16;
+    if (want > bytes) {
+      release();
+      GENPK_HIP(hipMalloc(&ptr, want));
+      bytes = want;
+    }
+    return ptr;
+  }
+  template <typename T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Deposit tiling.  The grid is cut into tiles of TX x TY x TZ cells
+// (clamped to d); each tile is stored with a one-cell halo on its high side
+// in x, y and z ("halo'd tile", HX x HY x HZ = (TX+1)(TY+1)(TZ+1) cells,
+// z fastest), so a tile receives every CIC corner of the particles whose
+// floor cell lies inside it.  Halo cells are folded into the neighbouring
+// tile's low face when the real-space grid is assembled.
+struct TileGeom {
+  int TX = 16, TY = 32, TZ = 32;
+  int ntx = 1, nty = 1, ntz = 1;
+  int HX = 17, HY = 33, HZ = 33;
+  int64_t K = 1;        // number of tiles
+  int64_t cells = 0;    // HX*HY*HZ
+  int kbits = 0;        // bits of the tile key
+  int passes = 1;       // radix passes (1 or 2)
+  int lo_bits = 0;      // bits of the low digit (pass 2), 0 if one pass
+  int hi_bits = 0;      // bits of the high digit (pass 1)
+};
+
+// Per-plan FFT twiddle tables.
+struct FftTables {
+  DevBuf tw_d_f64, tw_d_f32;      // exp(-2 pi i m / d), m < d
+  DevBuf tw_h_f64, tw_h_f32;      // exp(-2 pi i m / (d/2)), m < d/2
+};
+
+// Bin tables for one nrbins: q -> bin and the data-independent mode counts
+// and effective k.
+struct BinTables {
+  int nrbins = 0;
+  DevBuf q2bin;     // int32 [qmax+1], -1 for q = 0
+  DevBuf count;     // int32 [nrbins]
+  DevBuf keffsum;   // double [nrbins], unnormalised sum of h*|k|
+  bool counts_ready = false;
+};
+
+enum class FieldState { None, Real, Spectrum, YDone };
+
+struct Plan {
+  int64_t d = 0;
+  int prec = 8;             // bytes per real: 4 or 8
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int64_t hz = 0;           // d/2 + 1
+  int64_t rowlen = 0;       // 2*hz
+  size_t field_elems = 0;   // d*d*rowlen reals
+  TileGeom tg;
+
+  DevBuf field;             // padded grid, real or half-complex
+  FieldState fstate = FieldState::None;
+  DevBuf tiles;             // K * cells reals
+  bool tiles_live = false;  // tiles hold deposits not yet folded
+
+  // sort / deposit scratch
+  DevBuf recA, recB, table1, table2, seginfo, tile_start, item_start, part_start, heavy, partials;
+  DevBuf scan_tmp;
+
+  // window factors: float(onedinvwindow(KVAL(k), d)), k < d
+  DevBuf wtab;
+  FftTables fft;
+  std::map<int, std::unique_ptr<BinTables>> bins;
+  DevBuf bin_part;          // per-workgroup partial bins
+  DevBuf bin_tmp;           // power sums before normalisation
+
+  Plan(int64_t dims, int precision, hipStream_t s);
+};
+
+// deposit.hip
+void deposit(Plan &p, double boxsize, int64_t n, const float *pos, const float *masses, double mass);
+void fold_tiles(Plan &p, void *out, int64_t out_rowlen, bool accumulate);
+void tiles_to_field(Plan &p);
+// scan.hip
+void exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, DevBuf &tmp, hipStream_t s);
+// fft.hip
+void fft_tables_init(Plan &p);
+void r2c_zpass(Plan &p);
+void c2c_ypass(Plan &p, int sign);
+void c2c_xpass(Plan &p, int sign);
+void c2r_zpass(Plan &p);
+void r2c(Plan &p);
+void c2r(Plan &p);
+// bin.hip
+BinTables &bin_tables(Plan &p, int nrbins);
+void bin_spectra(Plan &a, Plan *b, int nrbins, bool fused_xpass, double *pow_aa, double *pow_bb,
+                 double *pow_ab, int *count, double *keffs, double ma, double mb);
+// adjoint.hip
+void adjoint_weight(Plan &p, int nrbins, const double *lambda, double m);
+void adjoint_gather(Plan &p, double boxsize, int64_t n, const float *pos, const float *masses, double mass,
+                    float *grad_pos, float *grad_mass);
+
+}  // namespace genpk

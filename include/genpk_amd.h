@@ -1,0 +1,121 @@
+/*
+ * genpk_amd -- MI355X (gfx950) implementation of GenPK's power-spectrum path:
+ * CIC deposit -> 3-D r2c FFT -> CIC-deconvolved log-binned P(k).
+ *
+ * Two groups of entry points, all C ABI, all returning 0 on success and a
+ * nonzero status on failure (message from genpk_last_error(); no C++
+ * exception crosses the boundary):
+ *
+ * 1. Device-resident plan API (stream ordered).  Pointers are device
+ *    pointers; work is enqueued on the plan's HIP stream and is complete
+ *    once that stream is synchronised.  A plan is not safe for concurrent
+ *    use from several host threads; distinct plans are independent.
+ *
+ * 2. Drop-in replacements of the reference's host-pointer call surface
+ *    (gen-pk.h), declared in genpk_compat.h.
+ */
+#ifndef GENPK_AMD_H
+#define GENPK_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Precision of the grid and FFT: bytes per real.  Bin sums are FP64 in
+ * both.  GENPK_FLOAT64 is the reference's precision (GENFLOAT = double,
+ * gen-pk.h:31). */
+#define GENPK_FLOAT32 4
+#define GENPK_FLOAT64 8
+
+typedef struct genpk_plan genpk_plan;
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char *genpk_last_error(void);
+
+/* Plan for a dims^3 grid.  dims: 2..4096 (the FFT needs an even dims whose
+ * half factors into 2, 3 and 5; deposit-only plans accept any dims >= 1).
+ * precision: GENPK_FLOAT32 or GENPK_FLOAT64.  stream: hipStream_t (NULL =
+ * default stream).  Replaces the reference's grid allocation and FFTW plan
+ * creation (gen-pk.cpp:169-193). */
+int genpk_plan_create(genpk_plan **plan, int64_t dims, int precision, void *stream);
+int genpk_plan_destroy(genpk_plan *plan);
+int genpk_plan_set_stream(genpk_plan *plan, void *stream);
+/* Bytes of the padded grid d*d*2(d/2+1) reals of the plan's precision. */
+int genpk_plan_field_bytes(const genpk_plan *plan, size_t *bytes);
+/* Device pointer of the plan's padded grid (real density or spectrum). */
+int genpk_plan_field_ptr(genpk_plan *plan, void **ptr);
+/* Make the plan work in a caller-owned device buffer of
+ * genpk_plan_field_bytes() bytes (e.g. a framework tensor) instead of its
+ * own grid; the caller keeps it alive until the plan is destroyed. */
+int genpk_plan_attach_field(genpk_plan *plan, void *ptr);
+
+/* Empty the density (the reference zeroes the grid, gen-pk.cpp:208). */
+int genpk_clear(genpk_plan *plan);
+
+/* CIC deposit of n particles, accumulating into the plan's density, as
+ * fieldize(boxsize, dims, field, n, pos, masses, mass, 1) does
+ * (fieldize.cpp:46-114).  pos: float [n][3]; masses: float [n] or NULL
+ * (then every particle has mass `mass`). */
+int genpk_deposit(genpk_plan *plan, double boxsize, int64_t n, const float *pos, const float *masses, double mass);
+
+/* Write the real-space density into out, a padded grid [d][d][out_rowlen]
+ * of the plan's precision; accumulate != 0 adds to out.  Pad columns
+ * (z >= d) are left untouched. */
+int genpk_export_density(genpk_plan *plan, void *out, int64_t out_rowlen, int accumulate);
+
+/* Replace the density by a caller grid [d][d][2(d/2+1)] of the plan's
+ * precision (pads ignored); later deposits add to it. */
+int genpk_import_density(genpk_plan *plan, const void *grid);
+
+/* In-place forward r2c of the density: the FFTW step of gen-pk.cpp:233.
+ * Afterwards the plan's grid holds the complex half spectrum
+ * [d][d][d/2+1] (unnormalised, sign -1). */
+int genpk_r2c(genpk_plan *plan);
+
+/* Unnormalised inverse (sign +1) of genpk_r2c on the plan's grid. */
+int genpk_c2r(genpk_plan *plan);
+
+/* Bin transformed spectra: powerspectrum(dims, Fa, Fb, nrbins, power,
+ * count, keffs, ma, mb) of powerspectrum.c:35-110.  b = NULL: auto
+ * spectrum of a into power_aa.  b != NULL: power_aa = P_aa (normalised by
+ * ma*ma), power_bb = P_bb (mb*mb), power_ab = Re(Fa Fb*) (ma*mb); any of
+ * the three may be NULL.  count (int [nrbins]) and keffs (double [nrbins])
+ * may be NULL.  All outputs are device pointers. */
+int genpk_bin(genpk_plan *a, genpk_plan *b, int nrbins, double *power_aa, double *power_bb, double *power_ab,
+              int *count, double *keffs, double ma, double mb);
+
+/* Fused forward: r2c of a (and b) whose last FFT pass bins the modes while
+ * they are in LDS; the spectrum is not kept (the plan's grid is left in an
+ * intermediate state).  Outputs as genpk_bin. */
+int genpk_power(genpk_plan *a, genpk_plan *b, int nrbins, double *power_aa, double *power_bb, double *power_ab,
+                int *count, double *keffs, double ma, double mb);
+
+/* Adjoint of the auto spectrum.  The plan must hold the forward spectrum
+ * (genpk_r2c).  lambda: device double [nrbins] = dL/dP_b; m: the total
+ * mass the forward normalised by.  Afterwards the plan's grid holds
+ * dL/drho on the padded real layout. */
+int genpk_adjoint_density(genpk_plan *plan, int nrbins, const double *lambda, double m);
+
+/* Chain dL/drho to the particles: grad_pos float [n][3] = dL/dpos,
+ * grad_mass float [n] = dL/dm_j (may be NULL).  pos, masses, mass, boxsize
+ * as given to genpk_deposit. */
+int genpk_adjoint_gather(genpk_plan *plan, double boxsize, int64_t n, const float *pos, const float *masses,
+                         double mass, float *grad_pos, float *grad_mass);
+
+/* Host-side utilities of the reference (utils.cpp:35-42, 57-72). */
+int genpk_nexttwo(int n);
+/* Writes the file-name token of a particle type ("by", "DM", "nu", "st",
+ * "xx") into out (>= 3 bytes). */
+int genpk_type_str(int type, char *out, size_t len);
+/* print_pk (utils.cpp:7-21): one "%e\t%e\t%d\n" line per bin with
+ * count > 0.  Host pointers. */
+int genpk_print_pk(const char *filename, int nrbins, const double *keffs, const double *power, const int *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GENPK_AMD_H */

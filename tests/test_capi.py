@@ -1,0 +1,86 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every symbol the
+public headers declare, and its host-only utilities match the reference's
+known answers (test.cpp:52-57, 102-113; utils.cpp:7-21).  No GPU calls."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import genpk_amd
+from genpk_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("genpk_amd.h", "genpk_compat.h")]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(\w+)\s*\(", text, flags=re.M):
+            name = m.group(1)
+            if name not in ("if", "defined"):
+                names.add(name)
+    return names
+
+
+def exported_symbols():
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True, check=True)
+    return {line.split()[-1] for line in out.stdout.splitlines() if line.strip()}
+
+
+def test_library_loads():
+    assert genpk_amd.lib() is not None
+
+
+def test_every_declared_symbol_exported():
+    decl = declared_functions()
+    assert {"genpk_plan_create", "genpk_deposit", "genpk_power", "powerspectrum", "invwindow", "fieldize"} <= decl
+    syms = exported_symbols()
+    for name in decl:
+        if name == "fieldize":
+            # C++ linkage, as gen-pk.h:93 declares it
+            assert "_Z8fieldizediPdlPfS0_di" in syms
+        else:
+            assert name in syms, name
+    assert set(_lib.SIGNATURES) <= syms
+
+
+def near(x, y):
+    return abs(x - y) <= max(abs(x), abs(y)) / 1e5
+
+
+def test_kat_invwindow():
+    # test.cpp:52-57
+    assert near(genpk_amd.invwindow(0, 3, 4, 5), 71.8177719)
+    assert near(genpk_amd.invwindow(4, 4, 4, 5), 6111.20801)
+    assert genpk_amd.invwindow(1, 1, 1, 0) == 0
+
+
+def test_invwindow_matches_oracle():
+    import oracle
+    for args in [(1, 2, 3, 16), (-7, 0, 8, 16), (100, 200, 300, 1024), (512, 512, 512, 1024), (0, 0, 0, 8)]:
+        assert genpk_amd.invwindow(*args) == oracle.invwindow(*args)
+
+
+def test_kat_type_str_and_nexttwo():
+    # test.cpp:102-113
+    assert genpk_amd.type_str(0) == "by"
+    assert genpk_amd.type_str(1) == "DM"
+    assert genpk_amd.type_str(2) == "nu" and genpk_amd.type_str(4) == "st" and genpk_amd.type_str(3) == "xx"
+    assert genpk_amd.nexttwo(12) == 16
+    assert genpk_amd.nexttwo(8) == 8
+    assert genpk_amd.nexttwo(1) == 1
+
+
+def test_print_pk_format(tmp_path):
+    k = np.array([0.5, 1.25, 2.0, 3.0])
+    p = np.array([1.0e-3, 0.0, 123456.789, 7.0])
+    c = np.array([6, 0, 12, 1], dtype=np.int32)
+    f = tmp_path / "PK-DM-test"
+    genpk_amd.print_pk(str(f), k, p, c)
+    expect = "".join("%e\t%e\t%d\n" % (k[i], p[i], c[i]) for i in range(4) if c[i])
+    assert f.read_text() == expect
