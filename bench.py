@@ -1,0 +1,209 @@
+#!/usr/bin/env python
+"""Throughput of the GenPK power-spectrum path on one MI355X.
+
+Metric: Gparticles/s of one forward pass -- CIC deposit + 3-D r2c FFT +
+CIC-deconvolved binned P(k) -- at a 1024^3 grid, on the clustered
+workload: 1024^3 particles in a 1000 Mpc/h box, half uniform and half in
+10^5 Gaussian clumps (power-law member counts, radii 0.5-4 cells), seeded,
+generated on the GPU (synthetic data).  Positions are resident in HBM
+before the timed region; one step = clear + deposit + fused r2c/bin.
+
+One JSON line is printed with the value, the roofline of the dominant
+kernel (HIP events recorded on the plan's stream around each stage during
+the timed steps), the whole-pipeline HBM fraction, the per-bin deviation of
+the timed precision from the FP64 path on the same particles, and the CPU
+oracle (the reference algorithm restated in C/OpenMP) timed on the host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--prec", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--n-log2", type=int, default=30, help="log2 of the particle count (1024^3 = 2^30)")
+    ap.add_argument("--dims", type=int, default=1024)
+    ap.add_argument("--box", type=float, default=1000.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 comparison pass")
+    ap.add_argument("--cpu-sample-log2", type=int, default=24)
+    return ap.parse_args()
+
+
+def stage_bytes(name, n, c, p_pad, cells_total, rec):
+    """Algorithmic HBM bytes of one launch of a stage: what the stage must
+    read and write once for its function (the design document lists them)."""
+    if name.startswith("sort_upsweep"):
+        return rec * n
+    if name.startswith("sort_downsweep"):
+        return 2 * rec * n
+    if name == "deposit_tiles":
+        return rec * n + c * cells_total
+    if name == "fold":
+        return c * cells_total + c * p_pad
+    if name in ("fft_z", "fft_y", "fft_x", "ifft_z", "ifft_y", "ifft_x"):
+        return 2 * c * p_pad
+    if name == "xpass_bin":
+        return c * p_pad
+    return 0
+
+
+def cpu_baseline(pos_dev, box, dims, nbins, n_total, sample_log2):
+    import numpy as np
+    import oracle
+    n = pos_dev.shape[0]
+    stride = max(1, n >> sample_log2)
+    sample = pos_dev[::stride].cpu().numpy()
+    ns = sample.shape[0]
+    grid = oracle.new_field(dims)
+    t0 = time.perf_counter()
+    oracle.fieldize(box, dims, grid, sample, None, 1.0, 1)
+    t_dep = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    F = oracle.fft_r2c_3d(grid)
+    t_fft = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.powerspectrum(dims, F, F, nbins, float(ns), float(ns))
+    t_bin = time.perf_counter() - t0
+    t_total = t_dep * (n_total / ns) + t_fft + t_bin
+    return {
+        "value": n_total / t_total / 1e9,
+        "unit": "Gparticles/s",
+        "cores": oracle.num_threads(),
+        "kind": "port",
+        "sample": (f"every {stride}th of the {n_total} particles ({ns}) deposited on the {dims}^3 FP64 grid "
+                   f"({t_dep:.2f} s, scaled x{n_total / ns:.0f}); FP64 r2c ({t_fft:.2f} s) and binning "
+                   f"({t_bin:.2f} s) at the full {dims}^3"),
+        "seconds_measured": round(t_dep + t_fft + t_bin, 2),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU decomposition is "
+              "described in DESIGN.md and not built.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import genpk_amd
+    from genpk_amd import synth
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    n = 1 << args.n_log2
+    d = args.dims
+    nbins = d
+    t_gen = time.perf_counter()
+    pos = synth.clumped_torch(n, args.box, d, n_clumps=100000, seed=4, device=dev)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t_gen
+
+    plan = genpk_amd.Plan(d, args.prec)
+
+    def step():
+        plan.clear()
+        plan.deposit(pos, args.box, mass=1.0)
+        return plan.power(nbins, mass=float(n))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    plan.timing_read()  # drop warm-up marks
+    plan.timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pw, cnt, keffs = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    plan.timing(False)
+    stages = plan.timing_read()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = n / (ms_step * 1e-3) / 1e9
+
+    c = 4 if args.prec == "f32" else 8
+    p_pad = d * d * 2 * (d // 2 + 1)
+    tg_cells = (-(-d // 16)) * (-(-d // 32)) * (-(-d // 32)) * 17 * 33 * 33
+    rec = 12
+    per_stage = {}
+    for name, (tot, launches) in stages.items():
+        avg = tot / max(launches, 1)
+        b = stage_bytes(name, n, c, p_pad, tg_cells, rec)
+        per_stage[name] = {"ms": round(avg, 4), "launches_per_step": launches / args.steps,
+                           "alg_bytes": b, "GBps": round(b / (avg * 1e-3) / 1e9, 1) if avg > 0 and b else None}
+    dom = max(per_stage, key=lambda k: per_stage[k]["ms"] * per_stage[k]["launches_per_step"])
+    ds = per_stage[dom]
+    achieved = ds["GBps"] or 0.0
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get(dom, {}).get(args.prec)
+        except Exception:
+            traffic = None
+    b_fwd = 12 * n + 4 * c * p_pad
+    pipeline_gbs = b_fwd / (ms_step * 1e-3) / 1e9
+
+    check = None
+    if not args.no_check:
+        pw_t = pw.double().cpu().numpy()
+        ref = genpk_amd.Plan(d, "f64")
+        ref.deposit(pos, args.box, mass=1.0)
+        pr, cr, kr = ref.power(nbins, mass=float(n))
+        pr = pr.cpu().numpy()
+        occ = cr.cpu().numpy() > 0
+        rel = np.abs(pw_t[occ] / pr[occ] - 1)
+        check = {"vs": "f64 path on the same particles", "max_rel": float(rel.max()),
+                 "median_rel": float(np.median(rel)), "bins": int(occ.sum()),
+                 "counts_equal": bool(np.array_equal(cnt.cpu().numpy(), cr.cpu().numpy()))}
+        del ref
+        torch.cuda.empty_cache()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(pos, args.box, d, nbins, n, args.cpu_sample_log2)
+
+    out = {
+        "metric": "Gparticles/s CIC+FFT+P(k) @1024^3",
+        "value": round(value, 3),
+        "unit": "Gparticles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.prec,
+        "data": "synthetic (seeded clustered particles generated on the GPU)",
+        "config": {"workload": f"clustered {n} particles (half uniform, half in 1e5 power-law clumps), "
+                               f"dims={d}, box={args.box} Mpc/h, forward CIC+r2c+P(k), nrbins={nbins}",
+                   "n_particles": n, "dims": d, "nrbins": nbins, "precision": args.prec},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel_ms": ds["ms"], "alg_bytes_per_launch": ds["alg_bytes"]},
+        "pipeline": {"alg_bytes": b_fwd, "GBps": round(pipeline_gbs, 1),
+                     "frac_of_hbm_peak": round(pipeline_gbs / HBM_PEAK_GBS, 4)},
+        "stages": per_stage,
+        "accuracy": check,
+        "cpu_baseline": cpu,
+        "input_gen_s": round(t_gen, 2),
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -170,6 +170,20 @@ class Plan:
                                 _ptr(count), _ptr(keffs), float(mass), float(mb)))
         return (tuple(pw) if cross else pw[0]), count, keffs
 
+    # -- timing ---------------------------------------------------------
+    def timing(self, enable=True):
+        check(lib().genpk_timing_enable(self._h, 1 if enable else 0))
+
+    def timing_read(self):
+        """{stage: (total_ms, launches)} accumulated since the last read."""
+        names = ctypes.create_string_buffer(4096)
+        ms = (ctypes.c_double * 64)()
+        cnt = (ctypes.c_int * 64)()
+        k = ctypes.c_int()
+        check(lib().genpk_timing_read(self._h, names, 4096, ms, cnt, 64, ctypes.byref(k)))
+        toks = names.raw.split(b"\0")
+        return {toks[i].decode(): (ms[i], cnt[i]) for i in range(k.value)}
+
     # -- adjoint --------------------------------------------------------
     def adjoint_density(self, lam, total_mass, nrbins=None):
         """dL/drho from dL/dP_b (the plan must hold the forward spectrum)."""

@@ -35,6 +35,8 @@ SIGNATURES = {
     "genpk_power": (_c_int, [_c_vp, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_dbl]),
     "genpk_adjoint_density": (_c_int, [_c_vp, _c_int, _c_vp, _c_dbl]),
     "genpk_adjoint_gather": (_c_int, [_c_vp, _c_dbl, _c_i64, _c_vp, _c_vp, _c_dbl, _c_vp, _c_vp]),
+    "genpk_timing_enable": (_c_int, [_c_vp, _c_int]),
+    "genpk_timing_read": (_c_int, [_c_vp, ctypes.c_char_p, _c_sz, _dp, _ip, _c_int, _ip]),
     "genpk_nexttwo": (_c_int, [_c_int]),
     "genpk_type_str": (_c_int, [_c_int, ctypes.c_char_p, _c_sz]),
     "genpk_print_pk": (_c_int, [ctypes.c_char_p, _c_int, _dp, _dp, _ip]),

@@ -92,6 +92,8 @@ void adjoint_weight(Plan &p, int nrbins, const double *lambda, double m) {
   const BinTables &bt = bin_tables(p, nrbins);
   const int64_t total = p.d * p.d * p.hz;
   const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(total, 256), 65536);
+  {
+  Stage st(p, "adjoint_weight");
   if (p.prec == 4)
     hipLaunchKernelGGL(k_weight<float>, dim3(grid), dim3(256), 0, p.stream, p.field.as<float2>(), p.d, p.hz,
                        bt.q2bin.as<int>(), bt.count.as<int>(), lambda, p.wtab.as<float>(), 1.0 / (m * m));
@@ -99,6 +101,7 @@ void adjoint_weight(Plan &p, int nrbins, const double *lambda, double m) {
     hipLaunchKernelGGL(k_weight<double>, dim3(grid), dim3(256), 0, p.stream, p.field.as<double2>(), p.d, p.hz,
                        bt.q2bin.as<int>(), bt.count.as<int>(), lambda, p.wtab.as<float>(), 1.0 / (m * m));
   GENPK_LAUNCH_CHECK();
+  }
   c2r(p);
 }
 
@@ -110,6 +113,7 @@ void adjoint_gather(Plan &p, double boxsize, int64_t n, const float *pos, const 
   GENPK_REQUIRE(pos && grad_pos, "null positions or gradient output");
   const double units = (double)p.d / boxsize;
   const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(n, 256), 1 << 20);
+  Stage st(p, "adjoint_gather");
   if (p.prec == 4) {
     if (masses)
       hipLaunchKernelGGL((k_gather<float, true>), dim3(grid), dim3(256), 0, p.stream, p.field.as<float>(), p.d,

@@ -383,6 +383,7 @@ static void launch_fused(Plan &a, Plan *b, const BinTables &bt, int B, bool keep
   const C *tw = (a.prec == 4) ? (const C *)a.fft.tw_d_f32.ptr : (const C *)a.fft.tw_d_f64.ptr;
   BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), a.wtab.as<float>()};
   const size_t lds = (size_t)NF * a.d * B * sizeof(C) + (size_t)((NS * bt.nrbins + 1) & ~1) * 8;
+  Stage st(a, "xpass_bin");
   hipLaunchKernelGGL((k_xpass_bin<T, NF>), dim3(G), dim3(BIN_THREADS), lds, a.stream, a.field.as<C>(),
                      b ? b->field.as<C>() : (C *)nullptr, g, B, nkb, rp, tw, keep ? 1 : 0, part);
   GENPK_LAUNCH_CHECK();
@@ -393,6 +394,7 @@ static void launch_stored(Plan &a, Plan *b, const BinTables &bt, double *part, i
   using C = typename Cx<T>::C;
   constexpr int NS = NF == 1 ? 1 : 3;
   BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), a.wtab.as<float>()};
+  Stage st(a, "bin_stored");
   hipLaunchKernelGGL((k_bin_stored<T, NF>), dim3(G), dim3(BIN_THREADS), (size_t)NS * bt.nrbins * 8, a.stream,
                      a.field.as<C>(), b ? b->field.as<C>() : (const C *)nullptr, g, part);
   GENPK_LAUNCH_CHECK();

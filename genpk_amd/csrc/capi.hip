@@ -279,6 +279,45 @@ int genpk_adjoint_gather(genpk_plan *plan, double boxsize, int64_t n, const floa
   });
 }
 
+int genpk_timing_enable(genpk_plan *plan, int enable) {
+  return guarded([&] {
+    GENPK_REQUIRE(plan && plan->p, "null plan");
+    plan->p->timer.enabled = enable != 0;
+  });
+}
+
+int genpk_timing_read(genpk_plan *plan, char *names, size_t names_len, double *ms, int *launches, int max,
+                      int *count) {
+  return guarded([&] {
+    GENPK_REQUIRE(plan && plan->p && names && ms && launches && count, "null argument");
+    Plan &p = *plan->p;
+    GENPK_HIP(hipStreamSynchronize(p.stream));
+    std::vector<std::string> order;
+    std::map<std::string, std::pair<double, int>> acc;
+    for (auto &m : p.timer.marks) {
+      float t = 0;
+      GENPK_HIP(hipEventElapsedTime(&t, m.a, m.b));
+      if (!acc.count(m.name)) order.push_back(m.name);
+      acc[m.name].first += t;
+      acc[m.name].second += 1;
+      p.timer.pool.push_back(m.a);
+      p.timer.pool.push_back(m.b);
+    }
+    p.timer.marks.clear();
+    size_t off = 0;
+    int k = 0;
+    for (auto &nm : order) {
+      if (k >= max || off + nm.size() + 1 > names_len) break;
+      std::memcpy(names + off, nm.c_str(), nm.size() + 1);
+      off += nm.size() + 1;
+      ms[k] = acc[nm].first;
+      launches[k] = acc[nm].second;
+      k++;
+    }
+    *count = k;
+  });
+}
+
 // utils.cpp:35-42
 int genpk_nexttwo(int n) {
   unsigned int i;

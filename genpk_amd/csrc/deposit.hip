@@ -514,13 +514,19 @@ void sort_and_deposit(Plan &p, double boxsize, int64_t n, const float *pos, cons
     uint32_t *t1 = static_cast<uint32_t *>(p.table1.ensure((size_t)nb * g1 * 4));
     DigitSel dsel{0, (uint32_t)(nb - 1)};
     ChunkMap cm{n, 0, nullptr, nullptr};
-    hipLaunchKernelGGL(k_upsweep<RW>, dim3((unsigned)g1), dim3(SORT_THREADS), nb * 4, s, raw, cm, kg, dsel, nb, t1,
-                       g1, (int64_t)1);
-    GENPK_LAUNCH_CHECK();
+    {
+      Stage st(p, "sort_upsweep1");
+      hipLaunchKernelGGL(k_upsweep<RW>, dim3((unsigned)g1), dim3(SORT_THREADS), nb * 4, s, raw, cm, kg, dsel, nb, t1,
+                         g1, (int64_t)1);
+      GENPK_LAUNCH_CHECK();
+    }
     exclusive_scan_u32(t1, t1, (int64_t)nb * g1, p.scan_tmp, s);
-    hipLaunchKernelGGL(k_downsweep<RW>, dim3((unsigned)g1), dim3(SORT_THREADS), downsweep_lds(nb, RW), s, raw, A, cm,
-                       kg, dsel, nb, tg.hi_bits, t1, g1, (int64_t)1);
-    GENPK_LAUNCH_CHECK();
+    {
+      Stage st(p, "sort_downsweep1");
+      hipLaunchKernelGGL(k_downsweep<RW>, dim3((unsigned)g1), dim3(SORT_THREADS), downsweep_lds(nb, RW), s, raw, A,
+                         cm, kg, dsel, nb, tg.hi_bits, t1, g1, (int64_t)1);
+      GENPK_LAUNCH_CHECK();
+    }
     tile_start = static_cast<uint32_t *>(p.tile_start.ensure((size_t)(ks + 1) * 4));
     hipLaunchKernelGGL(k_tile_start_1pass, dim3((unsigned)ceil_div(ks + 1, 256)), dim3(256), 0, s, t1, g1, (int)ks,
                        n, tile_start);
@@ -533,13 +539,19 @@ void sort_and_deposit(Plan &p, double boxsize, int64_t n, const float *pos, cons
     uint32_t *t1 = static_cast<uint32_t *>(p.table1.ensure((size_t)nhi * g1 * 4));
     DigitSel d1{tg.lo_bits, (uint32_t)(nhi - 1)};
     ChunkMap cm1{n, 0, nullptr, nullptr};
-    hipLaunchKernelGGL(k_upsweep<RW>, dim3((unsigned)g1), dim3(SORT_THREADS), nhi * 4, s, raw, cm1, kg, d1, nhi, t1,
-                       g1, (int64_t)1);
-    GENPK_LAUNCH_CHECK();
+    {
+      Stage st(p, "sort_upsweep1");
+      hipLaunchKernelGGL(k_upsweep<RW>, dim3((unsigned)g1), dim3(SORT_THREADS), nhi * 4, s, raw, cm1, kg, d1, nhi,
+                         t1, g1, (int64_t)1);
+      GENPK_LAUNCH_CHECK();
+    }
     exclusive_scan_u32(t1, t1, (int64_t)nhi * g1, p.scan_tmp, s);
-    hipLaunchKernelGGL(k_downsweep<RW>, dim3((unsigned)g1), dim3(SORT_THREADS), downsweep_lds(nhi, RW), s, raw, A,
-                       cm1, kg, d1, nhi, tg.hi_bits, t1, g1, (int64_t)1);
-    GENPK_LAUNCH_CHECK();
+    {
+      Stage st(p, "sort_downsweep1");
+      hipLaunchKernelGGL(k_downsweep<RW>, dim3((unsigned)g1), dim3(SORT_THREADS), downsweep_lds(nhi, RW), s, raw, A,
+                         cm1, kg, d1, nhi, tg.hi_bits, t1, g1, (int64_t)1);
+      GENPK_LAUNCH_CHECK();
+    }
     // segments of pass 2
     uint32_t *seg = static_cast<uint32_t *>(p.seginfo.ensure((size_t)(nhi + 1) * 3 * 4));
     uint32_t *seg_start = seg, *nchunks = seg + (nhi + 1), *chunk_start = seg + 2 * (nhi + 1);
@@ -553,9 +565,12 @@ void sort_and_deposit(Plan &p, double boxsize, int64_t n, const float *pos, cons
     DigitSel d2{0, (uint32_t)(nlo - 1)};
     ChunkMap cm2{n, nhi, seg_start, chunk_start};
     Src<RW> pa{A, nullptr, false};
-    hipLaunchKernelGGL(k_upsweep<RW>, dim3((unsigned)g2), dim3(SORT_THREADS), nlo * 4, s, pa, cm2, kg, d2, nlo, t2,
-                       (int64_t)1, (int64_t)nlo);
-    GENPK_LAUNCH_CHECK();
+    {
+      Stage st(p, "sort_upsweep2");
+      hipLaunchKernelGGL(k_upsweep<RW>, dim3((unsigned)g2), dim3(SORT_THREADS), nlo * 4, s, pa, cm2, kg, d2, nlo, t2,
+                         (int64_t)1, (int64_t)nlo);
+      GENPK_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(k_seg_total, dim3((unsigned)ceil_div(ks, 256)), dim3(256), 0, s, t2, chunk_start, nhi, nlo,
                        tot);
     GENPK_LAUNCH_CHECK();
@@ -566,9 +581,12 @@ void sort_and_deposit(Plan &p, double boxsize, int64_t n, const float *pos, cons
     hipLaunchKernelGGL(k_seg_offsets, dim3((unsigned)ceil_div(ks, 256)), dim3(256), 0, s, t2, chunk_start, nhi, nlo,
                        tile_start);
     GENPK_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_downsweep<RW>, dim3((unsigned)g2), dim3(SORT_THREADS), downsweep_lds(nlo, RW), s, pa, B,
-                       cm2, kg, d2, nlo, tg.lo_bits, t2, (int64_t)1, (int64_t)nlo);
-    GENPK_LAUNCH_CHECK();
+    {
+      Stage st(p, "sort_downsweep2");
+      hipLaunchKernelGGL(k_downsweep<RW>, dim3((unsigned)g2), dim3(SORT_THREADS), downsweep_lds(nlo, RW), s, pa, B,
+                         cm2, kg, d2, nlo, tg.lo_bits, t2, (int64_t)1, (int64_t)nlo);
+      GENPK_LAUNCH_CHECK();
+    }
     sorted = B;
   }
 
@@ -598,6 +616,7 @@ void sort_and_deposit(Plan &p, double boxsize, int64_t n, const float *pos, cons
   P.mass = mass;
   P.store = store;
   const size_t lds = (size_t)tg.cells * 8;
+  Stage st(p, "deposit_tiles");
   if (p.prec == 4) {
     hipLaunchKernelGGL((k_deposit<float, RW>), dim3((unsigned)max_items), dim3(DEP_THREADS), lds, s, sorted,
                        tile_start, item_start, part_start, P, p.tiles.as<float>(), partials);
@@ -660,6 +679,7 @@ void fold_tiles(Plan &p, void *out, int64_t out_rowlen, bool accumulate) {
   const unsigned gy = (unsigned)std::min<int64_t>(lines, 65535);
   const unsigned gz = (unsigned)ceil_div(lines, gy);
   dim3 grid((unsigned)ceil_div(p.d, FOLD_THREADS), gy, gz);
+  Stage st(p, "fold");
   if (p.prec == 4)
     hipLaunchKernelGGL((k_fold<float, float>), grid, dim3(FOLD_THREADS), 0, p.stream, p.tiles.as<float>(), F,
                        static_cast<float *>(out));

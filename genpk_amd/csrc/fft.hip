@@ -205,6 +205,7 @@ static void r2c_z_impl(Plan &p) {
   const RadixPlan rp = make_radix_plan((int)M);
   const C *twM = (p.prec == 4) ? (const C *)p.fft.tw_h_f32.ptr : (const C *)p.fft.tw_h_f64.ptr;
   const C *twd = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
+  Stage st(p, "fft_z");
   hipLaunchKernelGGL(k_r2c_z<T>, dim3((unsigned)ceil_div(nrows, L)), dim3(FFT_THREADS), (size_t)L * M * sizeof(C),
                      p.stream, p.field.as<T>(), p.rowlen, nrows, L, rp, twM, twd);
   GENPK_LAUNCH_CHECK();
@@ -219,6 +220,7 @@ static void c2r_z_impl(Plan &p) {
   const RadixPlan rp = make_radix_plan((int)M);
   const C *twM = (p.prec == 4) ? (const C *)p.fft.tw_h_f32.ptr : (const C *)p.fft.tw_h_f64.ptr;
   const C *twd = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
+  Stage st(p, "ifft_z");
   hipLaunchKernelGGL(k_c2r_z<T>, dim3((unsigned)ceil_div(nrows, L)), dim3(FFT_THREADS),
                      (size_t)L * (M + 1) * sizeof(C), p.stream, p.field.as<T>(), p.rowlen, nrows, L, rp, twM, twd);
   GENPK_LAUNCH_CHECK();
@@ -231,6 +233,7 @@ static void cols_impl(Plan &p, int axis) {
   const int nkb = (int)ceil_div(p.hz, B);
   const RadixPlan rp = make_radix_plan((int)p.d);
   const C *tw = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
+  Stage st(p, axis == 1 ? (SIGN < 0 ? "fft_y" : "ifft_y") : (SIGN < 0 ? "fft_x" : "ifft_x"));
   hipLaunchKernelGGL((k_cols<T, SIGN>), dim3((unsigned)(p.d * nkb)), dim3(FFT_THREADS), (size_t)p.d * B * sizeof(C),
                      p.stream, p.field.as<C>(), p.d, p.hz, B, nkb, axis, rp, tw);
   GENPK_LAUNCH_CHECK();

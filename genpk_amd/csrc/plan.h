@@ -3,6 +3,7 @@
 #pragma once
 #include "common.h"
 #include <map>
+#include <string>
 #include <memory>
 #include <vector>
 
@@ -77,6 +78,32 @@ struct BinTables {
 
 enum class FieldState { None, Real, Spectrum, YDone };
 
+// Stage timing: hipEvents recorded on the plan's stream around each stage
+// while enabled; read back (and cleared) by genpk_timing_read().
+struct StageTimer {
+  bool enabled = false;
+  std::vector<hipEvent_t> pool;
+  struct Mark { std::string name; hipEvent_t a, b; };
+  std::vector<Mark> marks;
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    GENPK_HIP(hipEventCreate(&e));
+    return e;
+  }
+  ~StageTimer() {
+    for (auto &m : marks) {
+      (void)hipEventDestroy(m.a);
+      (void)hipEventDestroy(m.b);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
 struct Plan {
   int64_t d = 0;
   int prec = 8;             // bytes per real: 4 or 8
@@ -102,8 +129,29 @@ struct Plan {
   std::map<int, std::unique_ptr<BinTables>> bins;
   DevBuf bin_part;          // per-workgroup partial bins
   DevBuf bin_tmp;           // power sums before normalisation
+  StageTimer timer;
 
   Plan(int64_t dims, int precision, hipStream_t s);
+};
+
+// RAII stage scope: records start/stop events when timing is enabled.
+struct Stage {
+  Plan &p;
+  hipEvent_t a = nullptr, b = nullptr;
+  const char *name;
+  Stage(Plan &pl, const char *nm) : p(pl), name(nm) {
+    if (p.timer.enabled) {
+      a = p.timer.get();
+      b = p.timer.get();
+      GENPK_HIP(hipEventRecord(a, p.stream));
+    }
+  }
+  ~Stage() {
+    if (a) {
+      (void)hipEventRecord(b, p.stream);
+      p.timer.marks.push_back({name, a, b});
+    }
+  }
 };
 
 // deposit.hip

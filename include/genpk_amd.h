@@ -105,6 +105,16 @@ int genpk_adjoint_density(genpk_plan *plan, int nrbins, const double *lambda, do
 int genpk_adjoint_gather(genpk_plan *plan, double boxsize, int64_t n, const float *pos, const float *masses,
                          double mass, float *grad_pos, float *grad_mass);
 
+/* Stage timing (no counterpart in the reference, which has no timers):
+ * while enabled, hipEvents are recorded on the plan's stream around each
+ * stage (sort passes, tile deposit, fold, FFT passes, binning, adjoint).
+ * genpk_timing_read synchronises the stream, returns up to max entries of
+ * (stage name, total milliseconds, launches) accumulated since the last
+ * read and clears them; names are written NUL-separated into names. */
+int genpk_timing_enable(genpk_plan *plan, int enable);
+int genpk_timing_read(genpk_plan *plan, char *names, size_t names_len, double *ms, int *launches, int max,
+                      int *count);
+
 /* Host-side utilities of the reference (utils.cpp:35-42, 57-72). */
 int genpk_nexttwo(int n);
 /* Writes the file-name token of a particle type ("by", "DM", "nu", "st",
