@@ -168,6 +168,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
   double *bins = reinterpret_cast<double *>(lds_raw);
   C *buf = reinterpret_cast<C *>(bins + ((NS * g.nrbins + 1) & ~1));  // [NF][n][B], 16-byte aligned
   const int n = rp.n;
+  const int lgb = ilog2(B);
   for (int i = threadIdx.x; i < NS * g.nrbins; i += BIN_THREADS) bins[i] = 0;
   const int64_t d = g.d, hz = g.hz;
   const int64_t estride = d * hz;
@@ -183,18 +184,35 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
     const int64_t kz0 = (int64_t)kb * B;
     const int nb = (int)min((int64_t)B, hz - kz0);
     const int64_t off = y * hz + kz0;
-    __syncthreads();
-    for (int i = threadIdx.x; i < n * B; i += BIN_THREADS) {
-      const int e = i / B, b = i % B;
-      buf[i] = b < nb ? fa[off + e * estride + b] : C{0, 0};
-      if (NF == 2) buf[n * B + i] = b < nb ? fb[off + e * estride + b] : C{0, 0};
+    {
+      // register-staged loads: all global reads of the block are in flight together
+      constexpr int MAXE = BinCfg<T>::MAXE;
+      const int tot = n * B;
+      C va[MAXE], vb[NF == 2 ? MAXE : 1];
+#pragma unroll
+      for (int q = 0; q < MAXE; q++) {
+        const int i = threadIdx.x + q * BIN_THREADS;
+        const int e = i >> lgb, b = i & (B - 1);
+        const bool ok = i < tot && b < nb;
+        va[q] = ok ? fa[off + e * estride + b] : C{0, 0};
+        if (NF == 2) vb[NF == 2 ? q : 0] = ok ? fb[off + e * estride + b] : C{0, 0};
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < MAXE; q++) {
+        const int i = threadIdx.x + q * BIN_THREADS;
+        if (i < tot) {
+          buf[i] = va[q];
+          if (NF == 2) buf[tot + i] = vb[NF == 2 ? q : 0];
+        }
+      }
     }
     __syncthreads();
-    lds_fft<-1, true, BinCfg<T>::MAXE>(buf, rp, B, B, 1, tw);
-    if (NF == 2) lds_fft<-1, true, BinCfg<T>::MAXE>(buf + n * B, rp, B, B, 1, tw);
+    lds_fft<-1, true, BinCfg<T>::MAXE>(buf, rp, B, lgb, B, 1, tw);
+    if (NF == 2) lds_fft<-1, true, BinCfg<T>::MAXE>(buf + n * B, rp, B, lgb, B, 1, tw);
     if (keep) {
       for (int i = threadIdx.x; i < n * B; i += BIN_THREADS) {
-        const int e = i / B, b = i % B;
+        const int e = i >> lgb, b = i & (B - 1);
         if (b < nb) {
           fa[off + e * estride + b] = buf[i];
           if (NF == 2) fb[off + e * estride + b] = buf[n * B + i];

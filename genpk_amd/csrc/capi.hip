@@ -34,9 +34,19 @@ Plan::Plan(int64_t dims, int precision, hipStream_t s) {
   rowlen = 2 * hz;
   field_elems = (size_t)d * d * rowlen;
   TileGeom &t = tg;
-  t.TX = (int)std::min<int64_t>(16, d);
-  t.TY = (int)std::min<int64_t>(32, d);
-  t.TZ = (int)std::min<int64_t>(32, d);
+  // tile edges: powers of two, at most 16 x 32 x 32 and at most d
+  auto pow2_le = [](int64_t cap, int64_t v, int &lg) {
+    int64_t t = 1;
+    lg = 0;
+    while (t * 2 <= std::min(cap, v)) {
+      t *= 2;
+      lg++;
+    }
+    return (int)t;
+  };
+  t.TX = pow2_le(16, d, t.sx);
+  t.TY = pow2_le(32, d, t.sy);
+  t.TZ = pow2_le(32, d, t.sz);
   t.ntx = (int)ceil_div(d, t.TX);
   t.nty = (int)ceil_div(d, t.TY);
   t.ntz = (int)ceil_div(d, t.TZ);
@@ -45,6 +55,8 @@ Plan::Plan(int64_t dims, int precision, hipStream_t s) {
   t.HZ = t.TZ + 1;
   t.K = (int64_t)t.ntx * t.nty * t.ntz;
   t.cells = (int64_t)t.HX * t.HY * t.HZ;
+  t.cm = (int64_t)t.HX * t.HY * t.TZ;
+  t.cz = (int64_t)t.HX * t.HY;
   t.kbits = bits_for(t.K);
   if (t.kbits <= 9) {
     t.passes = 1;
@@ -254,7 +266,6 @@ int genpk_power(genpk_plan *a, genpk_plan *b, int nrbins, double *power_aa, doub
     (void)bin_tables(pa, nrbins);
     for (Plan *q : {&pa, pb}) {
       if (!q) continue;
-      tiles_to_field(*q);
       r2c_zpass(*q);
       c2c_ypass(*q, -1);
       q->fstate = FieldState::YDone;

@@ -17,6 +17,7 @@
 // u = (double)pos * (d/box); i0 = floor(u) wrapped into [0,d), i1 = i0+1
 // wrapped; f = u - floor(u); corner c has bit0 = x, bit1 = y, bit2 = z.
 #include "plan.h"
+#include "tiles.h"
 
 #include <algorithm>
 #include <cmath>
@@ -38,19 +39,25 @@ constexpr int MAX_DIGIT_BITS = 11;
 struct KeyGeom {
   double units;
   int d;
-  int TX, TY, TZ;
+  int TX, TY, TZ;      // tile edge, powers of two
+  int sx, sy, sz;      // log2 of the tile edges
   int ntx, nty, ntz;
 };
 
-// Floor cell of one coordinate, wrapped into [0, d), and the fractional part.
+// Floor cell of one coordinate, wrapped into [0, d), and the fractional part
+// (fieldize.cpp:66-75).  The floor is converted to a 32-bit integer as the
+// reference does; positions more than one box outside [0, box) take the
+// general modulo.
 __device__ __forceinline__ int cell_of(float p, double units, int d, double &frac) {
   const double u = (double)p * units;
   const double f = floor(u);
   frac = u - f;
-  long long fl = (long long)f;
-  long long w = fl % d;
-  if (w < 0) w += d;
-  return (int)w;
+  int c = (int)f;
+  if ((unsigned)c >= (unsigned)d) {
+    c %= d;
+    if (c < 0) c += d;
+  }
+  return c;
 }
 
 __device__ __forceinline__ uint32_t tile_key(const float *r, const KeyGeom &g) {
@@ -58,7 +65,14 @@ __device__ __forceinline__ uint32_t tile_key(const float *r, const KeyGeom &g) {
   const int cx = cell_of(r[0], g.units, g.d, fr);
   const int cy = cell_of(r[1], g.units, g.d, fr);
   const int cz = cell_of(r[2], g.units, g.d, fr);
-  return (uint32_t)(((cx / g.TX) * g.nty + (cy / g.TY)) * g.ntz + (cz / g.TZ));
+  return (uint32_t)((((cx >> g.sx) * g.nty + (cy >> g.sy)) * g.ntz) + (cz >> g.sz));
+}
+
+// 2^36 * w rounded to nearest, for w in [0, 1]: the integer lands in the low
+// mantissa bits of w * 2^36 + 2^52.
+__device__ __forceinline__ unsigned long long to_fixed(double w) {
+  const double t = fma(w, FIXED_ONE, 4503599627370496.0);
+  return (unsigned long long)__double_as_longlong(t) - 0x4330000000000000ull;
 }
 
 // Source of records: either the caller's arrays (positions [n][3] and an
@@ -118,15 +132,24 @@ template <int RW>
 __global__ void __launch_bounds__(SORT_THREADS) k_upsweep(Src<RW> src, ChunkMap cm, KeyGeom kg, DigitSel ds,
                                                           int nb, uint32_t *__restrict__ counts, int64_t sd,
                                                           int64_t sg) {
+  constexpr int UNR = 8;
   extern __shared__ uint32_t hist[];
   int64_t b, e;
   if (!cm.range(blockIdx.x, b, e)) return;
   for (int i = threadIdx.x; i < nb; i += SORT_THREADS) hist[i] = 0;
   __syncthreads();
-  for (int64_t i = b + threadIdx.x; i < e; i += SORT_THREADS) {
-    float r[4];
-    src.load(i, r);
-    atomicAdd(&hist[ds.of(tile_key(r, kg))], 1u);
+  for (int64_t base = b; base < e; base += SORT_THREADS * UNR) {
+    float r[UNR][4];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const int64_t i = base + u * SORT_THREADS + threadIdx.x;
+      if (i < e) src.load(i, r[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const int64_t i = base + u * SORT_THREADS + threadIdx.x;
+      if (i < e) atomicAdd(&hist[ds.of(tile_key(r[u], kg))], 1u);
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < nb; i += SORT_THREADS)
@@ -139,6 +162,11 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 }
 
 // Stable partition of each chunk by digit into dst at offsets[dg*sd + g*sg].
+// Per sub-chunk of SORT_SUB records (each wave owns 1024 consecutive ones):
+// wavefront ballots give every record its rank among equal digits, LDS
+// prefix sums turn ranks into positions of a digit-sorted staging copy, and
+// the staging copy is written out as contiguous runs.  The next sub-chunk's
+// records are loaded while the current one is written out.
 template <int RW>
 __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<RW> src, float *__restrict__ dst, ChunkMap cm,
                                                             KeyGeom kg, DigitSel ds, int nb, int dbits,
@@ -158,24 +186,26 @@ __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<RW> src, float *
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int i = threadIdx.x; i < nb; i += SORT_THREADS)
     gbase[i] = offsets[(int64_t)i * sd + (int64_t)blockIdx.x * sg];
+  for (int i = threadIdx.x; i < SORT_WAVES * nb; i += SORT_THREADS) whist[i] = 0;
+
+  float rec[SORT_PER_LANE][RW];
+  const int wbase = wave * (SORT_PER_LANE * 64) + lane;
+#pragma unroll
+  for (int it = 0; it < SORT_PER_LANE; it++) {
+    const int64_t i = cb + wbase + it * 64;
+    if (i < ce) src.load(i, rec[it]);
+  }
+  __syncthreads();
 
   for (int64_t s0 = cb; s0 < ce; s0 += SORT_SUB) {
-    const int64_t send = min(ce, s0 + (int64_t)SORT_SUB);
-    const int nsub = (int)(send - s0);
-    for (int i = threadIdx.x; i < SORT_WAVES * nb; i += SORT_THREADS) whist[i] = 0;
-    __syncthreads();
-    float rec[SORT_PER_LANE][RW];
+    const int nsub = (int)min((int64_t)SORT_SUB, ce - s0);
     uint32_t dig[SORT_PER_LANE];
     uint32_t rnk[SORT_PER_LANE];
 #pragma unroll
     for (int it = 0; it < SORT_PER_LANE; it++) {
-      const int loc = wave * (SORT_PER_LANE * 64) + it * 64 + lane;
+      const int loc = wbase + it * 64;
       const bool valid = loc < nsub;
-      uint32_t dg = 0;
-      if (valid) {
-        src.load(s0 + loc, rec[it]);
-        dg = ds.of(tile_key(rec[it], kg));
-      }
+      const uint32_t dg = valid ? ds.of(tile_key(rec[it], kg)) : 0u;
       uint64_t peers = __ballot(valid);
       for (int bit = 0; bit < dbits; bit++) {
         const bool set = (dg >> bit) & 1u;
@@ -183,19 +213,21 @@ __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<RW> src, float *
         peers &= set ? bb : ~bb;
       }
       dig[it] = dg;
+      rnk[it] = 0;
       if (valid) {
         const uint32_t before = whist[wave * nb + dg];
         rnk[it] = before + (uint32_t)__popcll(peers & lanemask_lt());
-        // all lanes have read the running count before the leader updates it
+        // every lane has read the running count before the group leader updates it
         __builtin_amdgcn_wave_barrier();
         if ((peers & lanemask_lt()) == 0) whist[wave * nb + dg] = before + (uint32_t)__popcll(peers);
       }
       __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    // per-digit exclusive prefix over waves, sub-chunk totals
+    // per-digit exclusive prefix over waves and sub-chunk totals
     for (int dgi = threadIdx.x; dgi < nb; dgi += SORT_THREADS) {
       uint32_t tot = 0;
+#pragma unroll
       for (int w = 0; w < SORT_WAVES; w++) {
         const uint32_t t = whist[w * nb + dgi];
         whist[w * nb + dgi] = tot;
@@ -204,7 +236,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<RW> src, float *
       subcnt[dgi] = tot;
     }
     __syncthreads();
-    // exclusive scan of subcnt -> lstart (block scan, nb <= 2048)
+    // exclusive scan of subcnt -> lstart (nb <= 2048)
     {
       const int per = (nb + SORT_THREADS - 1) / SORT_THREADS;
       const int first = threadIdx.x * per;
@@ -230,7 +262,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<RW> src, float *
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < SORT_PER_LANE; it++) {
-      const int loc = wave * (SORT_PER_LANE * 64) + it * 64 + lane;
+      const int loc = wbase + it * 64;
       if (loc < nsub) {
         const uint32_t dg = dig[it];
         const uint32_t p = lstart[dg] + whist[wave * nb + dg] + rnk[it];
@@ -240,6 +272,15 @@ __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<RW> src, float *
       }
     }
     __syncthreads();
+    // prefetch the next sub-chunk while this one is written out
+    {
+      const int64_t n0 = s0 + SORT_SUB;
+#pragma unroll
+      for (int it = 0; it < SORT_PER_LANE; it++) {
+        const int64_t i = n0 + wbase + it * 64;
+        if (i < ce) src.load(i, rec[it]);
+      }
+    }
     for (int p = threadIdx.x; p < nsub; p += SORT_THREADS) {
       const uint32_t dg = sdig[p];
       const int64_t dest = (int64_t)gbase[dg] + (p - (int64_t)lstart[dg]);
@@ -248,6 +289,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<RW> src, float *
     }
     __syncthreads();
     for (int i = threadIdx.x; i < nb; i += SORT_THREADS) gbase[i] += subcnt[i];
+    for (int i = threadIdx.x; i < SORT_WAVES * nb; i += SORT_THREADS) whist[i] = 0;
     __syncthreads();
   }
 }
@@ -334,7 +376,8 @@ __global__ void k_heavy_list(const uint32_t *__restrict__ pcnt, int64_t ks, uint
 struct DepParams {
   KeyGeom kg;
   int HX, HY, HZ;
-  int64_t cells;
+  int64_t cells;       // accumulator cells HX*HY*HZ
+  int64_t cm, cz, zoff;  // tile store layout
   int64_t ks;          // key space size (tile_start has ks+1 entries)
   double mass;         // constant mass (u64 path)
   int store;           // 1: write tiles, 0: add to tiles
@@ -347,6 +390,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
                                                          const uint32_t *__restrict__ part_start, DepParams P,
                                                          Real *__restrict__ tiles, double *__restrict__ partials) {
   using Acc = typename std::conditional<RW == 4, double, unsigned long long>::type;
+  constexpr int UNR = 4;
   extern __shared__ unsigned long long lds_raw[];
   Acc *acc = reinterpret_cast<Acc *>(lds_raw);
   const uint32_t item = blockIdx.x;
@@ -372,40 +416,80 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
   const int tz = (int)(k % ntz);
   const int ty = (int)((k / ntz) % nty);
   const int tx = (int)(k / ((int64_t)ntz * nty));
-  const int ox = tx * P.kg.TX, oy = ty * P.kg.TY, oz = tz * P.kg.TZ;
+  const int ox = tx << P.kg.sx, oy = ty << P.kg.sy, oz = tz << P.kg.sz;
   const int sy = P.HZ, sx = P.HY * P.HZ;
-  for (uint32_t i = pb + threadIdx.x; i < pe; i += DEP_THREADS) {
-    float r[RW];
+  float r[UNR][RW], nx[UNR][RW];
 #pragma unroll
-    for (int q = 0; q < RW; q++) r[q] = rec[(int64_t)i * RW + q];
-    double fx, fy, fz;
-    const int lx = cell_of(r[0], P.kg.units, P.kg.d, fx) - ox;
-    const int ly = cell_of(r[1], P.kg.units, P.kg.d, fy) - oy;
-    const int lz = cell_of(r[2], P.kg.units, P.kg.d, fz) - oz;
-    if ((unsigned)lx >= (unsigned)P.kg.TX || (unsigned)ly >= (unsigned)P.kg.TY || (unsigned)lz >= (unsigned)P.kg.TZ)
-      continue;  // cannot happen for records of this tile; guards the LDS bounds
-    const double wx[2] = {1.0 - fx, fx}, wy[2] = {1.0 - fy, fy}, wz[2] = {1.0 - fz, fz};
-    const int base = lx * sx + ly * sy + lz;
+  for (int u = 0; u < UNR; u++) {
+    const uint32_t i = pb + u * DEP_THREADS + threadIdx.x;
+    if (i < pe) {
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-      const int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
-      const double w = wx[bx] * wy[by] * wz[bz];
-      const int at = base + bx * sx + by * sy + bz;
-      if constexpr (RW == 4) {
-        atomicAdd(&acc[at], (double)r[3] * w);
-      } else {
-        atomicAdd(&acc[at], (unsigned long long)(w * FIXED_ONE + 0.5));
+      for (int q = 0; q < RW; q++) nx[u][q] = rec[(int64_t)i * RW + q];
+    }
+  }
+  for (uint32_t i0 = pb; i0 < pe; i0 += DEP_THREADS * UNR) {
+#pragma unroll
+    for (int u = 0; u < UNR; u++)
+#pragma unroll
+      for (int q = 0; q < RW; q++) r[u][q] = nx[u][q];
+    // loads of the next batch are issued before this batch is accumulated
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const uint32_t i = i0 + (UNR + u) * DEP_THREADS + threadIdx.x;
+      if (i < pe) {
+#pragma unroll
+        for (int q = 0; q < RW; q++) nx[u][q] = rec[(int64_t)i * RW + q];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const uint32_t i = i0 + u * DEP_THREADS + threadIdx.x;
+      if (i >= pe) continue;
+      double fx, fy, fz;
+      const int lx = cell_of(r[u][0], P.kg.units, P.kg.d, fx) - ox;
+      const int ly = cell_of(r[u][1], P.kg.units, P.kg.d, fy) - oy;
+      const int lz = cell_of(r[u][2], P.kg.units, P.kg.d, fz) - oz;
+      if ((unsigned)lx >= (unsigned)P.kg.TX || (unsigned)ly >= (unsigned)P.kg.TY ||
+          (unsigned)lz >= (unsigned)P.kg.TZ)
+        continue;  // cannot happen for records of this tile; guards the LDS bounds
+      const double gx = 1.0 - fx, gy = 1.0 - fy, gz = 1.0 - fz;
+      const double wxy[4] = {gx * gy, fx * gy, gx * fy, fx * fy};
+      const int base = lx * sx + ly * sy + lz;
+      const int off[4] = {0, sx, sy, sx + sy};
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const int at = base + off[c];
+        if constexpr (RW == 4) {
+          const double mw = (double)r[u][3] * wxy[c];
+          atomicAdd(&acc[at], mw * gz);
+          atomicAdd(&acc[at + 1], mw * fz);
+        } else {
+          atomicAdd(&acc[at], to_fixed(wxy[c] * gz));
+          atomicAdd(&acc[at + 1], to_fixed(wxy[c] * fz));
+        }
       }
     }
   }
   __syncthreads();
   const double scale = (RW == 4) ? 1.0 : P.mass * FIXED_INV;
   if (m == 1) {
-    Real *out = tiles + k * P.cells;
-    for (int64_t c = threadIdx.x; c < P.cells; c += DEP_THREADS) {
-      const double v = (double)acc[c] * scale;
-      if (P.store) out[c] = (Real)v;
-      else out[c] = (Real)((double)out[c] + v);
+    // rows of TZ values (aligned), then the z-halo plane
+    Real *rows = tiles + k * P.cm;
+    Real *zh = tiles + P.zoff + k * P.cz;
+    const int sz = P.kg.sz, TZ = P.kg.TZ;
+    if (P.store) {
+      for (int64_t c = threadIdx.x; c < P.cm; c += DEP_THREADS) {
+        const int64_t a = (c >> sz) * P.HZ + (c & (TZ - 1));
+        rows[c] = (Real)((double)acc[a] * scale);
+      }
+      for (int64_t c = threadIdx.x; c < P.cz; c += DEP_THREADS) zh[c] = (Real)((double)acc[c * P.HZ + TZ] * scale);
+    } else {
+      for (int64_t c = threadIdx.x; c < P.cm; c += DEP_THREADS) {
+        const int64_t a = (c >> sz) * P.HZ + (c & (TZ - 1));
+        rows[c] = (Real)((double)rows[c] + (double)acc[a] * scale);
+      }
+      for (int64_t c = threadIdx.x; c < P.cz; c += DEP_THREADS)
+        zh[c] = (Real)((double)zh[c] + (double)acc[c * P.HZ + TZ] * scale);
     }
   } else {
     double *out = partials + (int64_t)(part_start[k] + j) * P.cells;
@@ -419,8 +503,9 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const uint32_t *__restr
                                                          const uint32_t *__restrict__ heavy,
                                                          const uint32_t *__restrict__ item_start,
                                                          const uint32_t *__restrict__ part_start,
-                                                         const double *__restrict__ partials, int64_t cells,
-                                                         int store, Real *__restrict__ tiles) {
+                                                         const double *__restrict__ partials, DepParams P,
+                                                         Real *__restrict__ tiles) {
+  const int64_t cells = P.cells;
   const int64_t nblk = ceil_div(cells, 256);
   const int64_t units = (int64_t)(*nheavy) * nblk;
   for (int64_t u = blockIdx.x; u < units; u += gridDim.x) {
@@ -431,54 +516,61 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const uint32_t *__restr
     const double *pp = partials + (int64_t)part_start[k] * cells + c;
     double s = 0;
     for (uint32_t j = 0; j < m; j++) s += pp[(int64_t)j * cells];
-    Real *out = tiles + (int64_t)k * cells + c;
-    *out = store ? (Real)s : (Real)((double)*out + s);
+    const int64_t row = c / P.HZ;
+    const int lz = (int)(c - row * P.HZ);
+    Real *out = lz < P.kg.TZ ? tiles + (int64_t)k * P.cm + row * P.kg.TZ + lz : tiles + P.zoff + (int64_t)k * P.cz + row;
+    *out = P.store ? (Real)s : (Real)((double)*out + s);
+  }
+}
+
+// Halo fold: every cell on a tile's low face (local x, y or z = 0) adds the
+// halo cells of the previous tiles that belong to it and zeroes them, so the
+// store holds each grid cell's full value at its own (tile, local) place.
+// Each halo cell maps to exactly one face cell, so the kernel is race free
+// and later accumulating deposits start from zero halos.
+template <typename Real>
+__global__ void __launch_bounds__(256) k_halo_fold(Real *__restrict__ tiles, FoldGeom P) {
+  const int64_t t = blockIdx.x;
+  const int tz = (int)(t % P.ntz);
+  const int ty = (int)((t / P.ntz) % P.nty);
+  const int tx = (int)(t / ((int64_t)P.ntz * P.nty));
+  const int wx = min(P.TX, P.d - tx * P.TX), wy = min(P.TY, P.d - ty * P.TY), wz = min(P.TZ, P.d - tz * P.TZ);
+  const int txp = tx == 0 ? P.ntx - 1 : tx - 1, typ = ty == 0 ? P.nty - 1 : ty - 1, tzp = tz == 0 ? P.ntz - 1 : tz - 1;
+  const int hx = min(P.TX, P.d - txp * P.TX), hy = min(P.TY, P.d - typ * P.TY), hz = min(P.TZ, P.d - tzp * P.TZ);
+  // face cells: plane lx = 0 (wy*wz), plane ly = 0 with lx >= 1 ((wx-1)*wz), line-free rest lz = 0 ((wx-1)(wy-1))
+  const int nA = wy * wz, nB = (wx - 1) * wz, nC = (wx - 1) * (wy - 1);
+  for (int f = threadIdx.x; f < nA + nB + nC; f += 256) {
+    int lx, ly, lz;
+    if (f < nA) { lx = 0; ly = f / wz; lz = f - ly * wz; }
+    else if (f < nA + nB) { const int g = f - nA; lx = 1 + g / wz; ly = 0; lz = g - (lx - 1) * wz; }
+    else { const int g = f - nA - nB; lx = 1 + g / (wy - 1); ly = 1 + g - (lx - 1) * (wy - 1); lz = 0; }
+    const int nxs = lx == 0 ? 2 : 1, nys = ly == 0 ? 2 : 1, nzs = lz == 0 ? 2 : 1;
+    double v = 0;
+    for (int a = 0; a < nxs; a++)
+      for (int b = 0; b < nys; b++)
+        for (int c = 0; c < nzs; c++) {
+          if (a == 0 && b == 0 && c == 0) continue;
+          const int64_t st = ((int64_t)(a ? txp : tx) * P.nty + (b ? typ : ty)) * P.ntz + (c ? tzp : tz);
+          const int64_t at = tile_at(P, st, a ? hx : lx, b ? hy : ly, c ? hz : lz);
+          v += (double)tiles[at];
+          tiles[at] = (Real)0;
+        }
+    const int64_t me = tile_at(P, t, lx, ly, lz);
+    tiles[me] = (Real)((double)tiles[me] + v);
   }
 }
 
 // Fold the halo'd tiles into a padded real grid [d][d][out_rowlen].
-struct FoldParams {
-  int d;
-  int TX, TY, TZ, ntx, nty, ntz, HX, HY, HZ;
-  int64_t cells;
-  int64_t out_rowlen;
-  int accumulate;
-};
-
 template <typename Real, typename Out>
-__global__ void __launch_bounds__(FOLD_THREADS) k_fold(const Real *__restrict__ tiles, FoldParams P,
-                                                       Out *__restrict__ out) {
+__global__ void __launch_bounds__(FOLD_THREADS) k_fold(const Real *__restrict__ tiles, FoldGeom P, int64_t out_rowlen,
+                                                       int accumulate, Out *__restrict__ out) {
   const int64_t line = blockIdx.y + (int64_t)blockIdx.z * gridDim.y;  // x*d + y
   const int z = blockIdx.x * FOLD_THREADS + threadIdx.x;
   if (z >= P.d || line >= (int64_t)P.d * P.d) return;
   const int x = (int)(line / P.d), y = (int)(line % P.d);
-  // sources along each axis: (tile, local index); two when on a tile's low face
-  int stx[2], slx[2], sty[2], sly[2], stz[2], slz[2];
-  int nxs = 1, nys = 1, nzs = 1;
-  stx[0] = x / P.TX; slx[0] = x - stx[0] * P.TX;
-  sty[0] = y / P.TY; sly[0] = y - sty[0] * P.TY;
-  stz[0] = z / P.TZ; slz[0] = z - stz[0] * P.TZ;
-  if (slx[0] == 0) {
-    const int t = (stx[0] + P.ntx - 1) % P.ntx;
-    stx[1] = t; slx[1] = min(P.TX, P.d - t * P.TX); nxs = 2;
-  }
-  if (sly[0] == 0) {
-    const int t = (sty[0] + P.nty - 1) % P.nty;
-    sty[1] = t; sly[1] = min(P.TY, P.d - t * P.TY); nys = 2;
-  }
-  if (slz[0] == 0) {
-    const int t = (stz[0] + P.ntz - 1) % P.ntz;
-    stz[1] = t; slz[1] = min(P.TZ, P.d - t * P.TZ); nzs = 2;
-  }
-  double v = 0;
-  for (int a = 0; a < nxs; a++)
-    for (int b = 0; b < nys; b++)
-      for (int c = 0; c < nzs; c++) {
-        const int64_t t = ((int64_t)stx[a] * P.nty + sty[b]) * P.ntz + stz[c];
-        v += (double)tiles[t * P.cells + ((int64_t)slx[a] * P.HY + sly[b]) * P.HZ + slz[c]];
-      }
-  Out *o = out + line * P.out_rowlen + z;
-  *o = P.accumulate ? (Out)((double)*o + v) : (Out)v;
+  const double v = (double)tiles[cell_at(P, x, y, z)];
+  Out *o = out + line * out_rowlen + z;
+  *o = accumulate ? (Out)((double)*o + v) : (Out)v;
 }
 
 KeyGeom key_geom(const Plan &p, double boxsize) {
@@ -486,6 +578,7 @@ KeyGeom key_geom(const Plan &p, double boxsize) {
   g.units = (double)p.d / boxsize;
   g.d = (int)p.d;
   g.TX = p.tg.TX; g.TY = p.tg.TY; g.TZ = p.tg.TZ;
+  g.sx = p.tg.sx; g.sy = p.tg.sy; g.sz = p.tg.sz;
   g.ntx = p.tg.ntx; g.nty = p.tg.nty; g.ntz = p.tg.ntz;
   return g;
 }
@@ -612,6 +705,9 @@ void sort_and_deposit(Plan &p, double boxsize, int64_t n, const float *pos, cons
   P.kg = kg;
   P.HX = tg.HX; P.HY = tg.HY; P.HZ = tg.HZ;
   P.cells = tg.cells;
+  P.cm = tg.cm;
+  P.cz = tg.cz;
+  P.zoff = tg.K * tg.cm;
   P.ks = ks;
   P.mass = mass;
   P.store = store;
@@ -622,25 +718,48 @@ void sort_and_deposit(Plan &p, double boxsize, int64_t n, const float *pos, cons
                        tile_start, item_start, part_start, P, p.tiles.as<float>(), partials);
     GENPK_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_reduce_partials<float>, dim3(1024), dim3(256), 0, s, nheavy, heavy, item_start, part_start,
-                       partials, tg.cells, store, p.tiles.as<float>());
+                       partials, P, p.tiles.as<float>());
   } else {
     hipLaunchKernelGGL((k_deposit<double, RW>), dim3((unsigned)max_items), dim3(DEP_THREADS), lds, s, sorted,
                        tile_start, item_start, part_start, P, p.tiles.as<double>(), partials);
     GENPK_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_reduce_partials<double>, dim3(1024), dim3(256), 0, s, nheavy, heavy, item_start, part_start,
-                       partials, tg.cells, store, p.tiles.as<double>());
+                       partials, P, p.tiles.as<double>());
   }
   GENPK_LAUNCH_CHECK();
+  {
+    Stage hst(p, "halo_fold");
+    const FoldGeom fg = fold_geom(p);
+    if (p.prec == 4)
+      hipLaunchKernelGGL(k_halo_fold<float>, dim3((unsigned)tg.K), dim3(256), 0, s, p.tiles.as<float>(), fg);
+    else
+      hipLaunchKernelGGL(k_halo_fold<double>, dim3((unsigned)tg.K), dim3(256), 0, s, p.tiles.as<double>(), fg);
+    GENPK_LAUNCH_CHECK();
+  }
 }
 
 }  // namespace
+
+FoldGeom fold_geom(const Plan &p) {
+  const TileGeom &tg = p.tg;
+  FoldGeom F;
+  F.d = (int)p.d;
+  F.TX = tg.TX; F.TY = tg.TY; F.TZ = tg.TZ;
+  F.sx = tg.sx; F.sy = tg.sy; F.sz = tg.sz;
+  F.ntx = tg.ntx; F.nty = tg.nty; F.ntz = tg.ntz;
+  F.HY = tg.HY; F.HZ = tg.HZ;
+  F.cm = tg.cm;
+  F.cz = tg.cz;
+  F.zoff = tg.K * tg.cm;
+  return F;
+}
 
 void deposit(Plan &p, double boxsize, int64_t n, const float *pos, const float *masses, double mass) {
   GENPK_REQUIRE(boxsize > 0 && std::isfinite(boxsize), "boxsize must be positive and finite");
   GENPK_REQUIRE(n >= 0, "negative particle count");
   GENPK_REQUIRE(n == 0 || pos != nullptr, "null positions");
   if (n == 0) return;  // adds nothing; an untouched density stays zero
-  p.tiles.ensure((size_t)p.tg.K * p.tg.cells * p.prec);
+  p.tiles.ensure((size_t)p.tg.K * (p.tg.cm + p.tg.cz) * p.prec);
   // one call handles at most 2^31 particles (32-bit record offsets)
   const int64_t cap = (int64_t)1 << 31;
   int64_t done = 0;
@@ -661,20 +780,12 @@ void deposit(Plan &p, double boxsize, int64_t n, const float *pos, const float *
 
 void fold_tiles(Plan &p, void *out, int64_t out_rowlen, bool accumulate) {
   GENPK_REQUIRE(out_rowlen >= p.d, "output row length shorter than dims");
-  const TileGeom &tg = p.tg;
   if (!p.tiles_live) {
     // nothing deposited: the density is zero
     if (!accumulate) GENPK_HIP(hipMemsetAsync(out, 0, (size_t)p.d * p.d * out_rowlen * p.prec, p.stream));
     return;
   }
-  FoldParams F;
-  F.d = (int)p.d;
-  F.TX = tg.TX; F.TY = tg.TY; F.TZ = tg.TZ;
-  F.ntx = tg.ntx; F.nty = tg.nty; F.ntz = tg.ntz;
-  F.HX = tg.HX; F.HY = tg.HY; F.HZ = tg.HZ;
-  F.cells = tg.cells;
-  F.out_rowlen = out_rowlen;
-  F.accumulate = accumulate ? 1 : 0;
+  const FoldGeom F = fold_geom(p);
   const int64_t lines = p.d * p.d;
   const unsigned gy = (unsigned)std::min<int64_t>(lines, 65535);
   const unsigned gz = (unsigned)ceil_div(lines, gy);
@@ -682,10 +793,10 @@ void fold_tiles(Plan &p, void *out, int64_t out_rowlen, bool accumulate) {
   Stage st(p, "fold");
   if (p.prec == 4)
     hipLaunchKernelGGL((k_fold<float, float>), grid, dim3(FOLD_THREADS), 0, p.stream, p.tiles.as<float>(), F,
-                       static_cast<float *>(out));
+                       out_rowlen, accumulate ? 1 : 0, static_cast<float *>(out));
   else
     hipLaunchKernelGGL((k_fold<double, double>), grid, dim3(FOLD_THREADS), 0, p.stream, p.tiles.as<double>(), F,
-                       static_cast<double *>(out));
+                       out_rowlen, accumulate ? 1 : 0, static_cast<double *>(out));
   GENPK_LAUNCH_CHECK();
 }
 

@@ -8,6 +8,7 @@
 //      as B-wide rows (coalesced) into LDS, transformed in place.
 #include "fft_core.h"
 #include "plan.h"
+#include "tiles.h"
 
 #include <cmath>
 #include <vector>
@@ -22,31 +23,56 @@ template <> struct FftCfg<float> { static constexpr int ELEMS = 8192; static con
 template <> struct FftCfg<double> { static constexpr int ELEMS = 4096; static constexpr int MAXE = 16; };
 
 // z pass, forward: rows of d reals -> d/2+1 complex, L rows per workgroup.
-template <typename T>
-__global__ void __launch_bounds__(FFT_THREADS) k_r2c_z(T *__restrict__ field, int64_t rowlen, int64_t nrows, int L,
-                                                      RadixPlan rp, const typename Cx<T>::C *__restrict__ twM,
+// FROM_TILES: the rows are assembled from the deposit's tile store while
+// loading (the fold), so the real-space grid is never written.
+// The LDS footprint (up to 64 KB) allows two workgroups per CU, so the
+// register budget of two waves per SIMD is available.
+template <typename T, bool FROM_TILES>
+__global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(1, 2))) k_r2c_z(T *__restrict__ field, const T *__restrict__ tiles,
+                                                      FoldGeom fg, FastDiv fdd, int64_t rowlen, int64_t nrows, int L,
+                                                      FastDiv fdM, FastDiv fdM1, RadixPlan rp,
+                                                      const typename Cx<T>::C *__restrict__ twM,
                                                       const typename Cx<T>::C *__restrict__ twd) {
   using C = typename Cx<T>::C;
+  constexpr int MAXE = FftCfg<T>::MAXE;
   extern __shared__ unsigned char lds_raw[];
   C *buf = reinterpret_cast<C *>(lds_raw);
   const int M = rp.n;
   const int64_t row0 = (int64_t)blockIdx.x * L;
   const int nl = (int)min((int64_t)L, nrows - row0);
-  for (int i = threadIdx.x; i < L * M; i += blockDim.x) {
-    const int l = i / M, m = i % M;
-    C v = C{0, 0};
-    if (l < nl) {
-      const T *row = field + (row0 + l) * rowlen;
-      v = C{row[2 * m], row[2 * m + 1]};
+  const int tot = L * M;
+  C v[MAXE];
+#pragma unroll
+  for (int q = 0; q < MAXE; q++) {
+    const int i = threadIdx.x + q * FFT_THREADS;
+    v[q] = C{0, 0};
+    if (i < tot) {
+      const int l = (int)fdM.div((uint32_t)i), m = i - l * M;
+      if (l < nl) {
+        const int64_t row = row0 + l;
+        if (FROM_TILES) {
+          // the tile store is halo-free: both reals sit in one aligned row (TZ even)
+          const int x = (int)fdd.div((uint32_t)row), y = (int)(row - (int64_t)x * fg.d);
+          const int64_t at = cell_at(fg, x, y, 2 * m);
+          v[q] = C{tiles[at], tiles[at + 1]};
+        } else {
+          const T *r = field + row * rowlen;
+          v[q] = C{r[2 * m], r[2 * m + 1]};
+        }
+      }
     }
-    buf[l * M + m] = v;
+  }
+#pragma unroll
+  for (int q = 0; q < MAXE; q++) {
+    const int i = threadIdx.x + q * FFT_THREADS;
+    if (i < tot) buf[i] = v[q];
   }
   __syncthreads();
-  lds_fft<-1, false, FftCfg<T>::MAXE>(buf, rp, L, 1, M, twM);
+  lds_fft<-1, false, MAXE>(buf, rp, L, 0, 1, M, twM);
   // X_k = (Z_k + conj Z_{M-k})/2 - i/2 * w^k (Z_k - conj Z_{M-k}),  w = exp(-2 pi i / d)
   const int nout = M + 1;
-  for (int i = threadIdx.x; i < nl * nout; i += blockDim.x) {
-    const int l = i / nout, k = i % nout;
+  for (int i = threadIdx.x; i < nl * nout; i += FFT_THREADS) {
+    const int l = (int)fdM1.div((uint32_t)i), k = i - l * nout;
     const C zk = buf[l * M + (k == M ? 0 : k)];
     const C zc = cconj(buf[l * M + (k == 0 ? 0 : M - k)]);
     const C e = C{(zk.x + zc.x) * (T)0.5, (zk.y + zc.y) * (T)0.5};
@@ -62,33 +88,34 @@ __global__ void __launch_bounds__(FFT_THREADS) k_r2c_z(T *__restrict__ field, in
 // imaginary parts of kz = 0 and kz = d/2 do not contribute.
 template <typename T>
 __global__ void __launch_bounds__(FFT_THREADS) k_c2r_z(T *__restrict__ field, int64_t rowlen, int64_t nrows, int L,
-                                                      RadixPlan rp, const typename Cx<T>::C *__restrict__ twM,
+                                                      FastDiv fdM, FastDiv fdM1, RadixPlan rp,
+                                                      const typename Cx<T>::C *__restrict__ twM,
                                                       const typename Cx<T>::C *__restrict__ twd) {
   using C = typename Cx<T>::C;
+  constexpr int MAXE = FftCfg<T>::MAXE;
   extern __shared__ unsigned char lds_raw[];
   C *buf = reinterpret_cast<C *>(lds_raw);  // [L][M+1]
   const int M = rp.n;
   const int W = M + 1;
   const int64_t row0 = (int64_t)blockIdx.x * L;
   const int nl = (int)min((int64_t)L, nrows - row0);
-  for (int i = threadIdx.x; i < L * W; i += blockDim.x) {
-    const int l = i / W, k = i % W;
+  for (int i = threadIdx.x; i < L * W; i += FFT_THREADS) {
+    const int l = (int)fdM1.div((uint32_t)i), k = i - l * W;
     C v = C{0, 0};
     if (l < nl) {
       const T *row = field + (row0 + l) * rowlen;
       v = C{row[2 * k], (k == 0 || k == M) ? (T)0 : row[2 * k + 1]};
     }
-    buf[l * W + k] = v;
+    buf[i] = v;
   }
   __syncthreads();
   // Z_k = (X_k + conj X_{M-k}) + i w^-k (X_k - conj X_{M-k}), k < M; staged in registers
-  constexpr int MAXE = FftCfg<T>::MAXE;
   C z[MAXE];
 #pragma unroll
   for (int q = 0; q < MAXE; q++) {
-    const int i = threadIdx.x + q * blockDim.x;
+    const int i = threadIdx.x + q * FFT_THREADS;
     if (i < L * M) {
-      const int l = i / M, k = i % M;
+      const int l = (int)fdM.div((uint32_t)i), k = i - l * M;
       const C xk = buf[l * W + k];
       const C xc = cconj(buf[l * W + (M - k)]);
       const C s = cadd(xk, xc);
@@ -101,17 +128,14 @@ __global__ void __launch_bounds__(FFT_THREADS) k_c2r_z(T *__restrict__ field, in
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < MAXE; q++) {
-    const int i = threadIdx.x + q * blockDim.x;
-    if (i < L * M) {
-      const int l = i / M, k = i % M;
-      buf[l * M + k] = z[q];
-    }
+    const int i = threadIdx.x + q * FFT_THREADS;
+    if (i < L * M) buf[i] = z[q];
   }
   __syncthreads();
-  lds_fft<+1, false, MAXE>(buf, rp, L, 1, M, twM);
-  for (int i = threadIdx.x; i < nl * M; i += blockDim.x) {
-    const int l = i / M, m = i % M;
-    const C v = buf[l * M + m];
+  lds_fft<+1, false, MAXE>(buf, rp, L, 0, 1, M, twM);
+  for (int i = threadIdx.x; i < nl * M; i += FFT_THREADS) {
+    const int l = (int)fdM.div((uint32_t)i), m = i - l * M;
+    const C v = buf[i];
     T *row = field + (row0 + l) * rowlen;
     row[2 * m] = v.x;
     row[2 * m + 1] = v.y;
@@ -122,30 +146,44 @@ __global__ void __launch_bounds__(FFT_THREADS) k_c2r_z(T *__restrict__ field, in
   }
 }
 
-// c2c along y (axis = 1) or x (axis = 0) over column blocks of B kz values.
+// c2c along y (axis = 1) or x (axis = 0) over column blocks of B = 1 << lgb
+// consecutive kz values.
 template <typename T, int SIGN>
 __global__ void __launch_bounds__(FFT_THREADS) k_cols(typename Cx<T>::C *__restrict__ f, int64_t d, int64_t hz,
-                                                     int B, int nkb, int axis, RadixPlan rp,
+                                                     int lgb, int nkb, int axis, RadixPlan rp,
                                                      const typename Cx<T>::C *__restrict__ tw) {
   using C = typename Cx<T>::C;
+  constexpr int MAXE = FftCfg<T>::MAXE;
   extern __shared__ unsigned char lds_raw[];
   C *buf = reinterpret_cast<C *>(lds_raw);  // [n][B]
   const int n = rp.n;
+  const int B = 1 << lgb;
   const int64_t outer = blockIdx.x / nkb;
-  const int kb = blockIdx.x % nkb;
+  const int kb = blockIdx.x - (int)(outer * nkb);
   const int64_t kz0 = (int64_t)kb * B;
   const int nb = (int)min((int64_t)B, hz - kz0);
   const int64_t estride = axis == 1 ? hz : d * hz;
   C *base = f + (axis == 1 ? outer * d * hz : outer * hz) + kz0;
-  for (int i = threadIdx.x; i < n * B; i += blockDim.x) {
-    const int e = i / B, b = i % B;
-    buf[i] = b < nb ? base[e * estride + b] : C{0, 0};
+  const int tot = n * B;
+  C v[MAXE];
+#pragma unroll
+  for (int q = 0; q < MAXE; q++) {
+    const int i = threadIdx.x + q * FFT_THREADS;
+    const int e = i >> lgb, b = i & (B - 1);
+    v[q] = (i < tot && b < nb) ? base[e * estride + b] : C{0, 0};
+  }
+#pragma unroll
+  for (int q = 0; q < MAXE; q++) {
+    const int i = threadIdx.x + q * FFT_THREADS;
+    if (i < tot) buf[i] = v[q];
   }
   __syncthreads();
-  lds_fft<SIGN, true, FftCfg<T>::MAXE>(buf, rp, B, B, 1, tw);
-  for (int i = threadIdx.x; i < n * B; i += blockDim.x) {
-    const int e = i / B, b = i % B;
-    if (b < nb) base[e * estride + b] = buf[i];
+  lds_fft<SIGN, true, MAXE>(buf, rp, B, lgb, B, 1, tw);
+#pragma unroll
+  for (int q = 0; q < MAXE; q++) {
+    const int i = threadIdx.x + q * FFT_THREADS;
+    const int e = i >> lgb, b = i & (B - 1);
+    if (i < tot && b < nb) base[e * estride + b] = buf[i];
   }
 }
 
@@ -197,7 +235,7 @@ void fft_tables_init(Plan &p) {
 }
 
 template <typename T>
-static void r2c_z_impl(Plan &p) {
+static void r2c_z_impl(Plan &p, bool from_tiles) {
   using C = typename Cx<T>::C;
   const int64_t M = p.d / 2;
   const int L = rows_per_block<T>(M);
@@ -205,9 +243,20 @@ static void r2c_z_impl(Plan &p) {
   const RadixPlan rp = make_radix_plan((int)M);
   const C *twM = (p.prec == 4) ? (const C *)p.fft.tw_h_f32.ptr : (const C *)p.fft.tw_h_f64.ptr;
   const C *twd = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
-  Stage st(p, "fft_z");
-  hipLaunchKernelGGL(k_r2c_z<T>, dim3((unsigned)ceil_div(nrows, L)), dim3(FFT_THREADS), (size_t)L * M * sizeof(C),
-                     p.stream, p.field.as<T>(), p.rowlen, nrows, L, rp, twM, twd);
+  const FoldGeom fg = fold_geom(p);
+  const dim3 grid((unsigned)ceil_div(nrows, L));
+  const size_t lds = (size_t)L * M * sizeof(C);
+  if (from_tiles) {
+    Stage st(p, "fold_fft_z");
+    hipLaunchKernelGGL((k_r2c_z<T, true>), grid, dim3(FFT_THREADS), lds, p.stream, p.field.as<T>(),
+                       p.tiles.as<const T>(), fg, FastDiv((uint32_t)p.d), p.rowlen, nrows, L, FastDiv((uint32_t)M),
+                       FastDiv((uint32_t)(M + 1)), rp, twM, twd);
+  } else {
+    Stage st(p, "fft_z");
+    hipLaunchKernelGGL((k_r2c_z<T, false>), grid, dim3(FFT_THREADS), lds, p.stream, p.field.as<T>(),
+                       (const T *)nullptr, fg, FastDiv((uint32_t)p.d), p.rowlen, nrows, L, FastDiv((uint32_t)M),
+                       FastDiv((uint32_t)(M + 1)), rp, twM, twd);
+  }
   GENPK_LAUNCH_CHECK();
 }
 
@@ -222,7 +271,8 @@ static void c2r_z_impl(Plan &p) {
   const C *twd = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
   Stage st(p, "ifft_z");
   hipLaunchKernelGGL(k_c2r_z<T>, dim3((unsigned)ceil_div(nrows, L)), dim3(FFT_THREADS),
-                     (size_t)L * (M + 1) * sizeof(C), p.stream, p.field.as<T>(), p.rowlen, nrows, L, rp, twM, twd);
+                     (size_t)L * (M + 1) * sizeof(C), p.stream, p.field.as<T>(), p.rowlen, nrows, L,
+                     FastDiv((uint32_t)M), FastDiv((uint32_t)(M + 1)), rp, twM, twd);
   GENPK_LAUNCH_CHECK();
 }
 
@@ -235,12 +285,18 @@ static void cols_impl(Plan &p, int axis) {
   const C *tw = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
   Stage st(p, axis == 1 ? (SIGN < 0 ? "fft_y" : "ifft_y") : (SIGN < 0 ? "fft_x" : "ifft_x"));
   hipLaunchKernelGGL((k_cols<T, SIGN>), dim3((unsigned)(p.d * nkb)), dim3(FFT_THREADS), (size_t)p.d * B * sizeof(C),
-                     p.stream, p.field.as<C>(), p.d, p.hz, B, nkb, axis, rp, tw);
+                     p.stream, p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, axis, rp, tw);
   GENPK_LAUNCH_CHECK();
 }
 
 void r2c_zpass(Plan &p) {
-  if (p.prec == 4) r2c_z_impl<float>(p); else r2c_z_impl<double>(p);
+  // a fresh deposit (no imported density) is folded while the z pass loads it
+  const bool from_tiles = p.tiles_live && p.fstate != FieldState::Real;
+  if (!from_tiles) tiles_to_field(p);
+  GENPK_REQUIRE(from_tiles || p.fstate == FieldState::Real, "r2c needs a real-space density");
+  if (p.prec == 4) r2c_z_impl<float>(p, from_tiles); else r2c_z_impl<double>(p, from_tiles);
+  p.tiles_live = false;
+  p.fstate = FieldState::Real;
 }
 void c2r_zpass(Plan &p) {
   if (p.prec == 4) c2r_z_impl<float>(p); else c2r_z_impl<double>(p);
@@ -261,8 +317,6 @@ void c2c_xpass(Plan &p, int sign) {
 }
 
 void r2c(Plan &p) {
-  tiles_to_field(p);
-  GENPK_REQUIRE(p.fstate == FieldState::Real, "r2c needs a real-space density");
   r2c_zpass(p);
   c2c_ypass(p, -1);
   c2c_xpass(p, -1);

@@ -108,23 +108,70 @@ template <int SIGN, typename C> struct Dft<5, SIGN, C> {
   }
 };
 
-// Radix sequence of a length.
+// Division by a runtime constant d (< 2^31) with a multiply-high, an add
+// and a shift (Granlund-Montgomery round-up method), exact for n < 2^31.
+struct FastDiv {
+  uint32_t d = 1, mul = 0, shift = 0;
+  FastDiv() = default;
+  explicit FastDiv(uint32_t dv) : d(dv) {
+    if (dv <= 1) return;
+    uint32_t l = 0;
+    while ((1ull << l) < dv) l++;
+    const unsigned __int128 one = 1;
+    const unsigned __int128 m = ((one << (32 + l)) + dv - 1) / dv;
+    mul = (uint32_t)(m - (one << 32));
+    shift = l - 1;
+  }
+  __host__ __device__ __forceinline__ uint32_t div(uint32_t n) const {
+    if (d <= 1) return n;
+#ifdef __HIP_DEVICE_COMPILE__
+    const uint32_t t = __umulhi(n, mul);
+#else
+    const uint32_t t = (uint32_t)(((uint64_t)n * mul) >> 32);
+#endif
+    return (t + ((n - t) >> 1)) >> shift;
+  }
+};
+
+// One Stockham stage: radix R, ns = product of the previous radices,
+// nq = n / R butterflies per transform, twstep = n / (ns R).
+struct StagePlan {
+  int R = 1, ns = 1, nq = 1, twstep = 1;
+  FastDiv fns, fnq;
+};
+
+// Radix sequence of a length with its per-stage index dividers.
 struct RadixPlan {
   int n = 1;
   int nst = 0;
-  int r[16] = {0};
+  StagePlan st[14];
 };
 
 inline RadixPlan make_radix_plan(int n) {
   RadixPlan p;
   p.n = n;
-  int m = n;
-  while (m % 8 == 0) { p.r[p.nst++] = 8; m /= 8; }
-  while (m % 4 == 0) { p.r[p.nst++] = 4; m /= 4; }
-  while (m % 2 == 0) { p.r[p.nst++] = 2; m /= 2; }
-  while (m % 3 == 0) { p.r[p.nst++] = 3; m /= 3; }
-  while (m % 5 == 0) { p.r[p.nst++] = 5; m /= 5; }
-  if (m != 1) p.nst = -1;  // unsupported prime factor
+  int r[32], nr = 0, m = n;
+  while (m % 8 == 0) { r[nr++] = 8; m /= 8; }
+  while (m % 4 == 0) { r[nr++] = 4; m /= 4; }
+  while (m % 2 == 0) { r[nr++] = 2; m /= 2; }
+  while (m % 3 == 0) { r[nr++] = 3; m /= 3; }
+  while (m % 5 == 0) { r[nr++] = 5; m /= 5; }
+  if (m != 1 || nr > 14) {
+    p.nst = -1;  // unsupported prime factor
+    return p;
+  }
+  int ns = 1;
+  for (int s = 0; s < nr; s++) {
+    StagePlan &sp = p.st[s];
+    sp.R = r[s];
+    sp.ns = ns;
+    sp.nq = n / r[s];
+    sp.twstep = n / (ns * r[s]);
+    sp.fns = FastDiv((uint32_t)ns);
+    sp.fnq = FastDiv((uint32_t)sp.nq);
+    ns *= r[s];
+  }
+  p.nst = nr;
   return p;
 }
 
@@ -133,32 +180,41 @@ inline bool fft_length_ok(int64_t n) {
   return make_radix_plan((int)n).nst >= 0;
 }
 
-// One Stockham stage of radix R on nbatch transforms of length n held in
-// LDS.  MAXE bounds the elements a thread owns per stage.  BFAST: the
-// batch index varies fastest across threads (interleaved layout).
+// One Stockham stage of radix R on nbatch transforms held in LDS, element
+// e of transform b at buf[e*se + b*sb].  MAXE bounds the elements a thread
+// owns per stage.  BFAST: the batch index varies fastest across threads
+// (interleaved layout, nbatch = 1 << lgb); otherwise the element index does.
 template <int R, int SIGN, bool BFAST, int MAXE, typename C>
-__device__ __forceinline__ void fft_stage(C *buf, int n, int nbatch, int se, int sb, int ns,
+__device__ __forceinline__ void fft_stage(C *buf, int nbatch, int lgb, int se, int sb, const StagePlan &sp,
                                           const C *__restrict__ tw) {
   constexpr int MAXB = (MAXE + R - 1) / R;
-  const int nq = n / R;
+  const int nq = sp.nq, ns = sp.ns;
   const int nbf = nbatch * nq;
-  const int twstep = n / (ns * R);
   C v[MAXB][R];
+  int dsto[MAXB];
 #pragma unroll
   for (int i = 0; i < MAXB; i++) {
     const int u = threadIdx.x + i * blockDim.x;
     if (u < nbf) {
-      const int b = BFAST ? u % nbatch : u / nq;
-      const int j = BFAST ? u / nbatch : u % nq;
-      const int k = j % ns;
-      C *src = buf + b * sb + j * se;
+      int b, j;
+      if (BFAST) {
+        b = u & (nbatch - 1);
+        j = u >> lgb;
+      } else {
+        b = (int)sp.fnq.div((uint32_t)u);
+        j = u - b * nq;
+      }
+      const int jq = (int)sp.fns.div((uint32_t)j);
+      const int k = j - jq * ns;
+      const C *src = buf + b * sb + j * se;
       v[i][0] = src[0];
 #pragma unroll
       for (int r = 1; r < R; r++) {
-        C t = tw[r * k * twstep];
+        C t = tw[r * k * sp.twstep];
         if (SIGN > 0) t.y = -t.y;
         v[i][r] = cmul(src[r * nq * se], t);
       }
+      dsto[i] = b * sb + (jq * ns * R + k) * se;
     }
   }
   __syncthreads();
@@ -166,11 +222,8 @@ __device__ __forceinline__ void fft_stage(C *buf, int n, int nbatch, int se, int
   for (int i = 0; i < MAXB; i++) {
     const int u = threadIdx.x + i * blockDim.x;
     if (u < nbf) {
-      const int b = BFAST ? u % nbatch : u / nq;
-      const int j = BFAST ? u / nbatch : u % nq;
-      const int k = j % ns;
       Dft<R, SIGN, C>::run(v[i]);
-      C *dst = buf + b * sb + ((j / ns) * ns * R + k) * se;
+      C *dst = buf + dsto[i];
 #pragma unroll
       for (int r = 0; r < R; r++) dst[r * ns * se] = v[i][r];
     }
@@ -180,19 +233,25 @@ __device__ __forceinline__ void fft_stage(C *buf, int n, int nbatch, int se, int
 
 // Full transform; the caller has filled buf and synchronised.
 template <int SIGN, bool BFAST, int MAXE, typename C>
-__device__ void lds_fft(C *buf, const RadixPlan &rp, int nbatch, int se, int sb, const C *__restrict__ tw) {
-  int ns = 1;
+__device__ __forceinline__ void lds_fft(C *buf, const RadixPlan &rp, int nbatch, int lgb, int se, int sb,
+                        const C *__restrict__ tw) {
   for (int s = 0; s < rp.nst; s++) {
-    switch (rp.r[s]) {
-      case 8: fft_stage<8, SIGN, BFAST, MAXE>(buf, rp.n, nbatch, se, sb, ns, tw); break;
-      case 4: fft_stage<4, SIGN, BFAST, MAXE>(buf, rp.n, nbatch, se, sb, ns, tw); break;
-      case 2: fft_stage<2, SIGN, BFAST, MAXE>(buf, rp.n, nbatch, se, sb, ns, tw); break;
-      case 3: fft_stage<3, SIGN, BFAST, MAXE>(buf, rp.n, nbatch, se, sb, ns, tw); break;
-      case 5: fft_stage<5, SIGN, BFAST, MAXE>(buf, rp.n, nbatch, se, sb, ns, tw); break;
+    const StagePlan &sp = rp.st[s];
+    switch (sp.R) {
+      case 8: fft_stage<8, SIGN, BFAST, MAXE>(buf, nbatch, lgb, se, sb, sp, tw); break;
+      case 4: fft_stage<4, SIGN, BFAST, MAXE>(buf, nbatch, lgb, se, sb, sp, tw); break;
+      case 2: fft_stage<2, SIGN, BFAST, MAXE>(buf, nbatch, lgb, se, sb, sp, tw); break;
+      case 3: fft_stage<3, SIGN, BFAST, MAXE>(buf, nbatch, lgb, se, sb, sp, tw); break;
+      case 5: fft_stage<5, SIGN, BFAST, MAXE>(buf, nbatch, lgb, se, sb, sp, tw); break;
       default: break;
     }
-    ns *= rp.r[s];
   }
+}
+
+__host__ __device__ inline int ilog2(int64_t v) {
+  int l = 0;
+  while (((int64_t)1 << l) < v) l++;
+  return l;
 }
 
 }  // namespace genpk

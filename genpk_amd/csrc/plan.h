@@ -49,11 +49,14 @@ struct DevBuf {
 // floor cell lies inside it.  Halo cells are folded into the neighbouring
 // tile's low face when the real-space grid is assembled.
 struct TileGeom {
-  int TX = 16, TY = 32, TZ = 32;
+  int TX = 16, TY = 32, TZ = 32;   // powers of two
+  int sx = 4, sy = 5, sz = 5;      // log2 of TX, TY, TZ
   int ntx = 1, nty = 1, ntz = 1;
   int HX = 17, HY = 33, HZ = 33;
   int64_t K = 1;        // number of tiles
-  int64_t cells = 0;    // HX*HY*HZ
+  int64_t cells = 0;    // HX*HY*HZ: accumulator cells of one tile
+  int64_t cm = 0;       // HX*HY*TZ: row-store values of one tile
+  int64_t cz = 0;       // HX*HY: z-halo values of one tile
   int kbits = 0;        // bits of the tile key
   int passes = 1;       // radix passes (1 or 2)
   int lo_bits = 0;      // bits of the low digit (pass 2), 0 if one pass
@@ -116,7 +119,7 @@ struct Plan {
 
   DevBuf field;             // padded grid, real or half-complex
   FieldState fstate = FieldState::None;
-  DevBuf tiles;             // K * cells reals
+  DevBuf tiles;             // K*(cm + cz) reals: row store, then z-halo planes
   bool tiles_live = false;  // tiles hold deposits not yet folded
 
   // sort / deposit scratch
@@ -153,6 +156,9 @@ struct Stage {
     }
   }
 };
+
+struct FoldGeom;
+FoldGeom fold_geom(const Plan &p);
 
 // deposit.hip
 void deposit(Plan &p, double boxsize, int64_t n, const float *pos, const float *masses, double mass);
