@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 comparison pass")
     ap.add_argument("--cpu-sample-log2", type=int, default=24)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 4 GiB device copy (reference kernel for HBM byte counters)")
     return ap.parse_args()
 
 
@@ -171,6 +173,14 @@ def main():
                  "counts_equal": bool(np.array_equal(cnt.cpu().numpy(), cr.cpu().numpy()))}
         del ref
         torch.cuda.empty_cache()
+
+    if args.copy_calibration:
+        a = torch.empty(1 << 30, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     cpu = None
     if not args.no_cpu:

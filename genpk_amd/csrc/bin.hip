@@ -151,7 +151,23 @@ struct BinGeom {
   int64_t d, hz;
   int nrbins;
   const int *q2bin;
+  const int64_t *thr;
   const float *wtab;
+};
+
+// Bin of q, tracked along a run of modes: the table is read only when q
+// leaves the current bin's [thr[b], thr[b+1]) range.
+struct BinTrack {
+  int cur = -1;
+  int64_t lo = 0, hi = 0;
+  __device__ __forceinline__ int of(int64_t q, const BinGeom &g) {
+    if (cur < 0 || q < lo || q >= hi) {
+      cur = g.q2bin[q];
+      lo = g.thr[cur];
+      hi = g.thr[cur + 1];
+    }
+    return cur;
+  }
 };
 
 // Fused final FFT pass along x + deconvolved binning.  NF fields (1: auto,
@@ -169,6 +185,8 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
   C *buf = reinterpret_cast<C *>(bins + ((NS * g.nrbins + 1) & ~1));  // [NF][n][B], 16-byte aligned
   const int n = rp.n;
   const int lgb = ilog2(B);
+  C *stw = buf + (size_t)NF * n * B;  // twiddles staged in LDS
+  for (int i = threadIdx.x; i < n; i += BIN_THREADS) stw[i] = tw[i];
   for (int i = threadIdx.x; i < NS * g.nrbins; i += BIN_THREADS) bins[i] = 0;
   const int64_t d = g.d, hz = g.hz;
   const int64_t estride = d * hz;
@@ -208,8 +226,8 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
       }
     }
     __syncthreads();
-    lds_fft<-1, true, BinCfg<T>::MAXE>(buf, rp, B, lgb, B, 1, tw);
-    if (NF == 2) lds_fft<-1, true, BinCfg<T>::MAXE>(buf + n * B, rp, B, lgb, B, 1, tw);
+    lds_fft<-1, true, BinCfg<T>::MAXE>(buf, rp, B, lgb, B, 1, stw);
+    if (NF == 2) lds_fft<-1, true, BinCfg<T>::MAXE>(buf + n * B, rp, B, lgb, B, 1, stw);
     if (keep) {
       for (int i = threadIdx.x; i < n * B; i += BIN_THREADS) {
         const int e = i >> lgb, b = i & (B - 1);
@@ -224,14 +242,18 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
       const int64_t ky = kval(y, d);
       const double h = (kz == 0 || kz == d / 2) ? 1.0 : 2.0;
       const int64_t a = ky * ky + kz * kz;
+      const float wy = g.wtab[y], wz = g.wtab[kz];
       RunAcc r;
+      BinTrack bt;
       const int e1 = min(n, e0 + per);
       for (int e = e0; e < e1; e++) {
         const int64_t kx = kval(e, d);
         const int64_t q = a + kx * kx;
         if (q == 0) continue;
-        const int b = g.q2bin[q];
-        const double dc = deconv_of(g.wtab, e, y, kz);
+        const int b = bt.of(q, g);
+        const float gw = (g.wtab[e] * wy) * wz;   // the reference's ((wx*wy)*wz) in float
+        double dc = (double)gw * (double)gw;
+        dc *= dc;
         const C va = buf[e * B + my_b];
         const C vb = NF == 2 ? buf[n * B + e * B + my_b] : va;
         double c[3];
@@ -267,11 +289,12 @@ __global__ void __launch_bounds__(BIN_THREADS) k_bin_stored(const typename Cx<T>
     const int64_t kx = kval(x, d), ky = kval(y, d);
     const int64_t a = kx * kx + ky * ky;
     RunAcc r;
+    BinTrack bt;
     const int64_t z0 = lane * per, z1 = min(hz, z0 + per);
     for (int64_t kz = z0; kz < z1; kz++) {
       const int64_t q = a + kz * kz;
       if (q == 0) continue;
-      const int b = g.q2bin[q];
+      const int b = bt.of(q, g);
       const double h = (kz == 0 || kz == d / 2) ? 1.0 : 2.0;
       const double dc = deconv_of(g.wtab, x, y, kz);
       const C va = fa[row * hz + kz];
@@ -349,6 +372,19 @@ BinTables &bin_tables(Plan &p, int nrbins) {
   }
   bt->q2bin.ensure(tab.size() * 4);
   GENPK_HIP(hipMemcpy(bt->q2bin.ptr, tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
+  // thr[b] = smallest q >= 1 with bin(q) >= b (bins are non-decreasing in q)
+  std::vector<int64_t> thr((size_t)nrbins + 1, qmax + 1);
+  {
+    int64_t q = 1;
+    for (int b = 0; b <= nrbins; b++) {
+      while (q <= qmax && tab[q] < b) q++;
+      thr[b] = q;
+    }
+    for (int64_t q2 = 2; q2 <= qmax; q2++)
+      GENPK_REQUIRE(tab[q2] >= tab[q2 - 1], "bin index not monotonic in |k|^2");
+  }
+  bt->thr.ensure(thr.size() * 8);
+  GENPK_HIP(hipMemcpy(bt->thr.ptr, thr.data(), thr.size() * 8, hipMemcpyHostToDevice));
   if (!p.wtab.ptr) {
     std::vector<float> w(d);
     for (int64_t k = 0; k < d; k++) {
@@ -385,7 +421,7 @@ static int fused_block(int prec, int64_t d, int64_t hz, int nf, int ns, int nrbi
   const int elems = prec == 4 ? BinCfg<float>::ELEMS : BinCfg<double>::ELEMS;
   int B = col_block(prec, d, hz);
   while (B >= 1) {
-    const int64_t lds = (int64_t)nf * d * B * csize + (int64_t)((ns * nrbins + 1) & ~1) * 8;
+    const int64_t lds = ((int64_t)nf * B + 1) * d * csize + (int64_t)((ns * nrbins + 1) & ~1) * 8;
     if (lds <= LDS_LIMIT && d * B <= elems && BIN_THREADS % B == 0) return B;
     B /= 2;
   }
@@ -399,8 +435,8 @@ static void launch_fused(Plan &a, Plan *b, const BinTables &bt, int B, bool keep
   const int nkb = (int)ceil_div(a.hz, B);
   const RadixPlan rp = make_radix_plan((int)a.d);
   const C *tw = (a.prec == 4) ? (const C *)a.fft.tw_d_f32.ptr : (const C *)a.fft.tw_d_f64.ptr;
-  BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), a.wtab.as<float>()};
-  const size_t lds = (size_t)NF * a.d * B * sizeof(C) + (size_t)((NS * bt.nrbins + 1) & ~1) * 8;
+  BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), bt.thr.as<int64_t>(), a.wtab.as<float>()};
+  const size_t lds = ((size_t)NF * B + 1) * a.d * sizeof(C) + (size_t)((NS * bt.nrbins + 1) & ~1) * 8;
   Stage st(a, "xpass_bin");
   hipLaunchKernelGGL((k_xpass_bin<T, NF>), dim3(G), dim3(BIN_THREADS), lds, a.stream, a.field.as<C>(),
                      b ? b->field.as<C>() : (C *)nullptr, g, B, nkb, rp, tw, keep ? 1 : 0, part);
@@ -411,7 +447,7 @@ template <typename T, int NF>
 static void launch_stored(Plan &a, Plan *b, const BinTables &bt, double *part, int G) {
   using C = typename Cx<T>::C;
   constexpr int NS = NF == 1 ? 1 : 3;
-  BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), a.wtab.as<float>()};
+  BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), bt.thr.as<int64_t>(), a.wtab.as<float>()};
   Stage st(a, "bin_stored");
   hipLaunchKernelGGL((k_bin_stored<T, NF>), dim3(G), dim3(BIN_THREADS), (size_t)NS * bt.nrbins * 8, a.stream,
                      a.field.as<C>(), b ? b->field.as<C>() : (const C *)nullptr, g, part);

@@ -2,8 +2,8 @@
 //
 // 1. Tile sort.  Particles are partitioned by the tile of their floor cell
 //    with an MSD radix partition of one or two passes (upsweep histogram in
-//    LDS -> offset scan -> stable downsweep that ranks records with
-//    wavefront ballots, stages them in LDS and writes contiguous runs).
+//    LDS -> offset scan -> downsweep that ranks records with wavefront
+//    ballots, stages them in LDS by digit and writes contiguous runs).
 //    Global atomics are avoided: on this chip they run at ~27 Gop/s.
 // 2. Tile accumulation.  One workgroup per (tile, particle range) item
 //    accumulates the 8 CIC weights of each particle into the halo'd tile in
@@ -127,6 +127,11 @@ struct DigitSel {
   __device__ __forceinline__ uint32_t of(uint32_t key) const { return (key >> shift) & mask; }
 };
 
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const int lane = threadIdx.x & 63;
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
 // counts[dg * sd + g * sg] = number of records of chunk g with digit dg
 template <int RW>
 __global__ void __launch_bounds__(SORT_THREADS) k_upsweep(Src<RW> src, ChunkMap cm, KeyGeom kg, DigitSel ds,
@@ -148,17 +153,20 @@ __global__ void __launch_bounds__(SORT_THREADS) k_upsweep(Src<RW> src, ChunkMap 
 #pragma unroll
     for (int u = 0; u < UNR; u++) {
       const int64_t i = base + u * SORT_THREADS + threadIdx.x;
-      if (i < e) atomicAdd(&hist[ds.of(tile_key(r[u], kg))], 1u);
-    }
-  }
+      const bool valid = i < e;
+      const uint32_t dg = valid ? ds.of(tile_key(r[u], kg)) : 0u;
+      // a wavefront whose records share one digit adds them with one atomic
+      const uint64_t vmask = __ballot(valid);
+      const uint32_t d0 = __shfl(dg, (int)__builtin_ctzll(vmask | (1ull << 63)), 64);
+      if (__ballot(valid && dg == d0) == vmask) {
+        if (valid && (vmask & lanemask_lt()) == 0) atomicAdd(&hist[d0], (uint32_t)__popcll(vmask));
+      } else if (valid) {
+        atomicAdd(&hist[dg], 1u);
+      }
+    }  }
   __syncthreads();
   for (int i = threadIdx.x; i < nb; i += SORT_THREADS)
     counts[(int64_t)i * sd + (int64_t)blockIdx.x * sg] = hist[i];
-}
-
-__device__ __forceinline__ uint64_t lanemask_lt() {
-  const int lane = threadIdx.x & 63;
-  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
 // Stable partition of each chunk by digit into dst at offsets[dg*sd + g*sg].
@@ -207,10 +215,14 @@ __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<RW> src, float *
       const bool valid = loc < nsub;
       const uint32_t dg = valid ? ds.of(tile_key(rec[it], kg)) : 0u;
       uint64_t peers = __ballot(valid);
-      for (int bit = 0; bit < dbits; bit++) {
-        const bool set = (dg >> bit) & 1u;
-        const uint64_t bb = __ballot(set);
-        peers &= set ? bb : ~bb;
+      // wavefronts whose records share one digit skip the per-bit match
+      const uint32_t d0 = __shfl(dg, (int)__builtin_ctzll(peers | (1ull << 63)), 64);
+      if (__ballot(valid && dg == d0) != peers) {
+        for (int bit = 0; bit < dbits; bit++) {
+          const bool set = (dg >> bit) & 1u;
+          const uint64_t bb = __ballot(set);
+          peers &= set ? bb : ~bb;
+        }
       }
       dig[it] = dg;
       rnk[it] = 0;

@@ -67,8 +67,13 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
     const int i = threadIdx.x + q * FFT_THREADS;
     if (i < tot) buf[i] = v[q];
   }
+  // twiddles of the M-point FFT and of the even/odd split, staged in LDS
+  C *stw = buf + (size_t)L * M;   // [M]
+  C *sxw = stw + M;               // [M+1]
+  for (int i = threadIdx.x; i < M; i += FFT_THREADS) stw[i] = twM[i];
+  for (int i = threadIdx.x; i <= M; i += FFT_THREADS) sxw[i] = twd[i];
   __syncthreads();
-  lds_fft<-1, false, MAXE>(buf, rp, L, 0, 1, M, twM);
+  lds_fft<-1, false, MAXE>(buf, rp, L, 0, 1, M, stw);
   // X_k = (Z_k + conj Z_{M-k})/2 - i/2 * w^k (Z_k - conj Z_{M-k}),  w = exp(-2 pi i / d)
   const int nout = M + 1;
   for (int i = threadIdx.x; i < nl * nout; i += FFT_THREADS) {
@@ -77,7 +82,7 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
     const C zc = cconj(buf[l * M + (k == 0 ? 0 : M - k)]);
     const C e = C{(zk.x + zc.x) * (T)0.5, (zk.y + zc.y) * (T)0.5};
     const C o = C{(zk.y - zc.y) * (T)0.5, -(zk.x - zc.x) * (T)0.5};  // (zk - zc) / (2i)
-    const C x = cadd(e, cmul(o, twd[k]));
+    const C x = cadd(e, cmul(o, sxw[k]));
     T *row = field + (row0 + l) * rowlen;
     row[2 * k] = x.x;
     row[2 * k + 1] = x.y;
@@ -131,8 +136,10 @@ __global__ void __launch_bounds__(FFT_THREADS) k_c2r_z(T *__restrict__ field, in
     const int i = threadIdx.x + q * FFT_THREADS;
     if (i < L * M) buf[i] = z[q];
   }
+  C *stw = buf + (size_t)L * (M + 1);
+  for (int i = threadIdx.x; i < M; i += FFT_THREADS) stw[i] = twM[i];
   __syncthreads();
-  lds_fft<+1, false, MAXE>(buf, rp, L, 0, 1, M, twM);
+  lds_fft<+1, false, MAXE>(buf, rp, L, 0, 1, M, stw);
   for (int i = threadIdx.x; i < nl * M; i += FFT_THREADS) {
     const int l = (int)fdM.div((uint32_t)i), m = i - l * M;
     const C v = buf[i];
@@ -177,8 +184,10 @@ __global__ void __launch_bounds__(FFT_THREADS) k_cols(typename Cx<T>::C *__restr
     const int i = threadIdx.x + q * FFT_THREADS;
     if (i < tot) buf[i] = v[q];
   }
+  C *stw = buf + tot;  // twiddles staged in LDS
+  for (int i = threadIdx.x; i < n; i += FFT_THREADS) stw[i] = tw[i];
   __syncthreads();
-  lds_fft<SIGN, true, MAXE>(buf, rp, B, lgb, B, 1, tw);
+  lds_fft<SIGN, true, MAXE>(buf, rp, B, lgb, B, 1, stw);
 #pragma unroll
   for (int q = 0; q < MAXE; q++) {
     const int i = threadIdx.x + q * FFT_THREADS;
@@ -245,7 +254,7 @@ static void r2c_z_impl(Plan &p, bool from_tiles) {
   const C *twd = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
   const FoldGeom fg = fold_geom(p);
   const dim3 grid((unsigned)ceil_div(nrows, L));
-  const size_t lds = (size_t)L * M * sizeof(C);
+  const size_t lds = ((size_t)L * M + 2 * M + 1) * sizeof(C);
   if (from_tiles) {
     Stage st(p, "fold_fft_z");
     hipLaunchKernelGGL((k_r2c_z<T, true>), grid, dim3(FFT_THREADS), lds, p.stream, p.field.as<T>(),
@@ -271,7 +280,7 @@ static void c2r_z_impl(Plan &p) {
   const C *twd = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
   Stage st(p, "ifft_z");
   hipLaunchKernelGGL(k_c2r_z<T>, dim3((unsigned)ceil_div(nrows, L)), dim3(FFT_THREADS),
-                     (size_t)L * (M + 1) * sizeof(C), p.stream, p.field.as<T>(), p.rowlen, nrows, L,
+                     ((size_t)L * (M + 1) + M) * sizeof(C), p.stream, p.field.as<T>(), p.rowlen, nrows, L,
                      FastDiv((uint32_t)M), FastDiv((uint32_t)(M + 1)), rp, twM, twd);
   GENPK_LAUNCH_CHECK();
 }
@@ -284,7 +293,7 @@ static void cols_impl(Plan &p, int axis) {
   const RadixPlan rp = make_radix_plan((int)p.d);
   const C *tw = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
   Stage st(p, axis == 1 ? (SIGN < 0 ? "fft_y" : "ifft_y") : (SIGN < 0 ? "fft_x" : "ifft_x"));
-  hipLaunchKernelGGL((k_cols<T, SIGN>), dim3((unsigned)(p.d * nkb)), dim3(FFT_THREADS), (size_t)p.d * B * sizeof(C),
+  hipLaunchKernelGGL((k_cols<T, SIGN>), dim3((unsigned)(p.d * nkb)), dim3(FFT_THREADS), (size_t)p.d * (B + 1) * sizeof(C),
                      p.stream, p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, axis, rp, tw);
   GENPK_LAUNCH_CHECK();
 }

@@ -74,6 +74,7 @@ struct FftTables {
 struct BinTables {
   int nrbins = 0;
   DevBuf q2bin;     // int32 [qmax+1], -1 for q = 0
+  DevBuf thr;       // int64 [nrbins+1]: smallest q >= 1 whose bin is >= b (qmax+1 past the end)
   DevBuf count;     // int32 [nrbins]
   DevBuf keffsum;   // double [nrbins], unnormalised sum of h*|k|
   bool counts_ready = false;
