@@ -49,6 +49,7 @@ class Plan:
         self.torch = torch
         self.dims = int(dims)
         self.prec = _precision(precision)
+        self.generation = 0  # bumped by every call that changes the plan's content
         self.device = torch.device("cuda", torch.cuda.current_device() if device is None else device)
         self._h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
@@ -101,13 +102,18 @@ class Plan:
 
     # -- forward --------------------------------------------------------
     def clear(self):
+        self.generation += 1
         check(lib().genpk_clear(self._h))
 
-    def deposit(self, pos, boxsize, masses=None, mass=1.0):
-        """Accumulating CIC deposit (fieldize.cpp:46-114)."""
+    def deposit(self, pos, boxsize, masses=None, mass=1.0, retain=False):
+        """Accumulating CIC deposit (fieldize.cpp:46-114).  retain: keep the
+        particles' tile order for a following adjoint_gather of the same
+        tensors."""
         self._check_particles(pos, masses)
         self._sync_stream()
-        check(lib().genpk_deposit(self._h, float(boxsize), pos.shape[0], _ptr(pos), _ptr(masses), float(mass)))
+        self.generation += 1
+        fn = lib().genpk_deposit_retain if retain else lib().genpk_deposit
+        check(fn(self._h, float(boxsize), pos.shape[0], _ptr(pos), _ptr(masses), float(mass)))
 
     def export_density(self, out=None, rowlen=None, accumulate=False):
         """Real-space density as a padded grid [d][d][rowlen] of the plan's precision."""
@@ -122,15 +128,24 @@ class Plan:
     def import_density(self, grid):
         assert grid.is_contiguous() and grid.dtype == self.real_dtype and grid.shape == self.grid.shape
         self._sync_stream()
+        self.generation += 1
         check(lib().genpk_import_density(self._h, _ptr(grid)))
+
+    def import_spectrum(self, spec):
+        """Load a half spectrum (complex tensor [d][d][d/2+1] of the plan's precision)."""
+        self._sync_stream()
+        self.generation += 1
+        check(lib().genpk_import_spectrum(self._h, _ptr(spec)))
 
     def r2c(self):
         self._sync_stream()
+        self.generation += 1
         check(lib().genpk_r2c(self._h))
         return self.spectrum()
 
     def c2r(self):
         self._sync_stream()
+        self.generation += 1
         check(lib().genpk_c2r(self._h))
         return self.density()
 
@@ -162,8 +177,10 @@ class Plan:
         cross = other is not None
         pw, count, keffs = self._outputs(nrbins, cross)
         self._sync_stream()
+        self.generation += 1
         if cross:
             other._sync_stream()
+            other.generation += 1
         mb = mass if other_mass is None else other_mass
         check(lib().genpk_power(self._h, other._h if cross else None, nrbins, _ptr(pw[0]),
                                 _ptr(pw[1]) if cross else None, _ptr(pw[2]) if cross else None,
@@ -186,11 +203,14 @@ class Plan:
 
     # -- adjoint --------------------------------------------------------
     def adjoint_density(self, lam, total_mass, nrbins=None):
-        """dL/drho from dL/dP_b (the plan must hold the forward spectrum)."""
+        """dL/drho from dL/dP_b.  The plan holds the forward spectrum, the
+        state left by power() (nothing deposited since), or the forward
+        density (the transform is then redone)."""
         nrbins = self.dims if nrbins is None else int(nrbins)
         lam = lam.to(dtype=self.torch.float64, device=self.device).contiguous()
         assert lam.shape == (nrbins,)
         self._sync_stream()
+        self.generation += 1
         check(lib().genpk_adjoint_density(self._h, nrbins, _ptr(lam), float(total_mass)))
         return self.density()
 

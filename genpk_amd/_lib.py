@@ -27,8 +27,10 @@ SIGNATURES = {
     "genpk_plan_attach_field": (_c_int, [_c_vp, _c_vp]),
     "genpk_clear": (_c_int, [_c_vp]),
     "genpk_deposit": (_c_int, [_c_vp, _c_dbl, _c_i64, _c_vp, _c_vp, _c_dbl]),
+    "genpk_deposit_retain": (_c_int, [_c_vp, _c_dbl, _c_i64, _c_vp, _c_vp, _c_dbl]),
     "genpk_export_density": (_c_int, [_c_vp, _c_vp, _c_i64, _c_int]),
     "genpk_import_density": (_c_int, [_c_vp, _c_vp]),
+    "genpk_import_spectrum": (_c_int, [_c_vp, _c_vp]),
     "genpk_r2c": (_c_int, [_c_vp]),
     "genpk_c2r": (_c_int, [_c_vp]),
     "genpk_bin": (_c_int, [_c_vp, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_dbl]),

@@ -9,127 +9,320 @@
 //   dL/dm_j  = sum_c w_jc G(c)
 //   dL/dx_ja = m_j (d/box) sum_c (dw_jc/du_a) G(c)
 // where w_jc are the CIC weights of fieldize.cpp:77-84 and G = dL/drho.
+//
+// dL/drho is formed from whichever forward state the plan holds: a kept
+// spectrum (weight, then the three inverse passes), the field after the
+// forward z and y passes (left there by the fused power), or a deposit or
+// density (z and y passes first).  In the last two cases the x pass runs
+// forward FFT -> weight -> inverse FFT on each column block in LDS, so the
+// spectrum is never written.  The gather visits the particles in tile order
+// (the deposit's partition, with each record's index), reads dL/drho from a
+// halo'd tile staged in LDS, and writes the gradients at the records' indices.
 #include "fft_core.h"
-#include "plan.h"
+#include "tilesort.h"
+
+#include <algorithm>
+#include <cmath>
 
 namespace genpk {
 
+int col_block(int prec, int64_t d, int64_t hz);
+
 namespace {
+
+constexpr int ADJ_THREADS = 256;
+constexpr int GATHER_THREADS = 512;
+template <typename T> struct AdjCfg;
+template <> struct AdjCfg<float> { static constexpr int MAXE = 32; };
+template <> struct AdjCfg<double> { static constexpr int MAXE = 16; };
 
 __device__ __forceinline__ int64_t kv(int64_t n, int64_t d) { return n <= d / 2 ? n : n - d; }
 
+struct AdjWeight {
+  const int *q2bin;
+  const int *count;
+  const double *lambda;
+  const float *wtab;
+  double inv_m2;
+  // 2 lambda_b g_k / (M^2 C_b) of the mode at array indices (x, y, kz); 0 at k = 0
+  __device__ __forceinline__ double at(int64_t x, int64_t y, int64_t kz, int64_t d) const {
+    const int64_t kx = kv(x, d), ky = kv(y, d);
+    const int64_t q = kx * kx + ky * ky + kz * kz;
+    if (q == 0) return 0.0;
+    const int b = q2bin[q];
+    const int c = count[b];
+    const float gw = (wtab[x] * wtab[y]) * wtab[kz];
+    double g = (double)gw * (double)gw;
+    g *= g;
+    return c ? 2.0 * lambda[b] * g * inv_m2 / (double)c : 0.0;
+  }
+};
+
 template <typename T>
-__global__ void k_weight(typename Cx<T>::C *__restrict__ f, int64_t d, int64_t hz, const int *__restrict__ q2bin,
-                         const int *__restrict__ count, const double *__restrict__ lambda,
-                         const float *__restrict__ wtab, double inv_m2) {
+__global__ void k_weight(typename Cx<T>::C *__restrict__ f, int64_t d, int64_t hz, AdjWeight w) {
   using C = typename Cx<T>::C;
   const int64_t total = d * d * hz;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t kz = i % hz;
     const int64_t row = i / hz;
-    const int64_t x = row / d, y = row % d;
-    const int64_t kx = kv(x, d), ky = kv(y, d);
-    const int64_t q = kx * kx + ky * ky + kz * kz;
-    C out = C{0, 0};
-    if (q != 0) {
-      const int b = q2bin[q];
-      const int c = count[b];
-      const float gw = (wtab[x] * wtab[y]) * wtab[kz];
-      double g = (double)gw * (double)gw;
-      g *= g;
-      const double s = c ? 2.0 * lambda[b] * g * inv_m2 / (double)c : 0.0;
-      const C v = f[i];
-      out = C{(T)((double)v.x * s), (T)((double)v.y * s)};
-    }
-    f[i] = out;
+    const double s = w.at(row / d, row % d, kz, d);
+    const C v = f[i];
+    f[i] = C{(T)((double)v.x * s), (T)((double)v.y * s)};
   }
 }
 
-template <typename T, bool MASSES>
-__global__ void k_gather(const T *__restrict__ G, int64_t d, int64_t rowlen, double units, int64_t n,
-                         const float *__restrict__ pos, const float *__restrict__ masses, double mass,
-                         float *__restrict__ gpos, float *__restrict__ gmass) {
-  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
-    int64_t i0[3], i1[3];
-    double f[3];
-    for (int a = 0; a < 3; a++) {
-      const double u = (double)pos[3 * j + a] * units;
-      const double fl = floor(u);
-      f[a] = u - fl;
-      long long c = (long long)fl % d;
-      if (c < 0) c += d;
-      i0[a] = c;
-      i1[a] = (c + 1) % d;
-    }
-    const double m = MASSES ? (double)masses[j] : mass;
-    double gm = 0, gx = 0, gy = 0, gz = 0;
+// x pass of the adjoint on the field after the forward z and y passes:
+// per column block (fixed y, B consecutive kz), forward length-d FFT along
+// x, multiply by the adjoint weight, inverse FFT, store back.
+template <typename T>
+__global__ void __launch_bounds__(ADJ_THREADS) k_adjoint_x(typename Cx<T>::C *__restrict__ f, int64_t d, int64_t hz,
+                                                          int lgb, int nkb, RadixPlan rp,
+                                                          const typename Cx<T>::C *__restrict__ tw, AdjWeight w) {
+  using C = typename Cx<T>::C;
+  constexpr int MAXE = AdjCfg<T>::MAXE;
+  extern __shared__ unsigned char lds_raw[];
+  C *buf = reinterpret_cast<C *>(lds_raw);  // [n][B]
+  const int n = rp.n;
+  const int B = 1 << lgb;
+  const int64_t y = blockIdx.x / nkb;
+  const int kb = blockIdx.x - (int)(y * nkb);
+  const int64_t kz0 = (int64_t)kb * B;
+  const int nb = (int)min((int64_t)B, hz - kz0);
+  const int64_t estride = d * hz;
+  C *base = f + y * hz + kz0;
+  const int tot = n * B;
+  C v[MAXE];
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-      const int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
-      const double wx = bx ? f[0] : 1.0 - f[0];
-      const double wy = by ? f[1] : 1.0 - f[1];
-      const double wz = bz ? f[2] : 1.0 - f[2];
-      const int64_t at = ((bx ? i1[0] : i0[0]) * d + (by ? i1[1] : i0[1])) * rowlen + (bz ? i1[2] : i0[2]);
-      const double g = (double)G[at];
-      gm += wx * wy * wz * g;
-      gx += (bx ? 1.0 : -1.0) * wy * wz * g;
-      gy += (by ? 1.0 : -1.0) * wx * wz * g;
-      gz += (bz ? 1.0 : -1.0) * wx * wy * g;
-    }
-    const double s = m * units;
-    gpos[3 * j + 0] = (float)(gx * s);
-    gpos[3 * j + 1] = (float)(gy * s);
-    gpos[3 * j + 2] = (float)(gz * s);
-    if (gmass) gmass[j] = (float)gm;
+  for (int q = 0; q < MAXE; q++) {
+    const int i = threadIdx.x + q * ADJ_THREADS;
+    const int e = i >> lgb, b = i & (B - 1);
+    v[q] = (i < tot && b < nb) ? base[e * estride + b] : C{0, 0};
   }
+#pragma unroll
+  for (int q = 0; q < MAXE; q++) {
+    const int i = threadIdx.x + q * ADJ_THREADS;
+    if (i < tot) buf[i] = v[q];
+  }
+  C *stw = buf + tot;
+  for (int i = threadIdx.x; i < n; i += ADJ_THREADS) stw[i] = tw[i];
+  __syncthreads();
+  lds_fft<-1, true, MAXE>(buf, rp, B, lgb, B, 1, stw);
+  __syncthreads();
+  for (int i = threadIdx.x; i < tot; i += ADJ_THREADS) {
+    const int e = i >> lgb, b = i & (B - 1);
+    if (b < nb) {
+      const double s = w.at(e, y, kz0 + b, d);
+      const C a = buf[i];
+      buf[i] = C{(T)((double)a.x * s), (T)((double)a.y * s)};
+    }
+  }
+  __syncthreads();
+  lds_fft<+1, true, MAXE>(buf, rp, B, lgb, B, 1, stw);
+#pragma unroll
+  for (int q = 0; q < MAXE; q++) {
+    const int i = threadIdx.x + q * ADJ_THREADS;
+    const int e = i >> lgb, b = i & (B - 1);
+    if (i < tot && b < nb) base[e * estride + b] = buf[i];
+  }
+}
+
+struct GatherParams {
+  KeyGeom kg;
+  int HY, HZ;
+  int64_t cells;
+  int64_t ks;
+  int64_t rowlen;
+  double mass;
+  FastDiv fhz, fhy;
+};
+
+// One workgroup per work item of the tile order: stage the halo'd tile of
+// G = dL/drho in LDS, then the gradients of the item's records.
+template <typename T, int NM>
+__global__ void __launch_bounds__(GATHER_THREADS) k_gather_tiles(const T *__restrict__ G, const float *__restrict__ rec,
+                                                                 const uint32_t *__restrict__ tile_start,
+                                                                 const uint32_t *__restrict__ item_start,
+                                                                 GatherParams P, float *__restrict__ gpos,
+                                                                 float *__restrict__ gmass) {
+  constexpr int RW = 4 + NM;
+  extern __shared__ unsigned char lds_raw[];
+  T *g = reinterpret_cast<T *>(lds_raw);
+  ItemRange ir;
+  if (!item_range(item_start, tile_start, P.ks, blockIdx.x, ir)) return;
+  if (ir.pb >= ir.pe) return;
+  const int64_t k = ir.k;
+  const int ntz = P.kg.ntz, nty = P.kg.nty, d = P.kg.d;
+  const int tz = (int)(k % ntz);
+  const int ty = (int)((k / ntz) % nty);
+  const int tx = (int)(k / ((int64_t)ntz * nty));
+  const int ox = tx << P.kg.sx, oy = ty << P.kg.sy, oz = tz << P.kg.sz;
+  for (int c = threadIdx.x; c < (int)P.cells; c += GATHER_THREADS) {
+    const int row = (int)P.fhz.div((uint32_t)c);
+    const int lz = c - row * P.HZ;
+    const int lx = (int)P.fhy.div((uint32_t)row);
+    const int ly = row - lx * P.HY;
+    int x = ox + lx, y = oy + ly, z = oz + lz;
+    if (x >= d) x -= d;
+    if (y >= d) y -= d;
+    if (z >= d) z -= d;
+    g[c] = G[((int64_t)x * d + y) * P.rowlen + z];
+  }
+  __syncthreads();
+  const int sy = P.HZ, sx = P.HY * P.HZ;
+  const double units = P.kg.units;
+  for (uint32_t i = ir.pb + threadIdx.x; i < ir.pe; i += GATHER_THREADS) {
+    float r[RW];
+#pragma unroll
+    for (int q = 0; q < RW; q++) r[q] = rec[(int64_t)i * RW + q];
+    double fx, fy, fz;
+    const int lx = cell_of(r[0], units, d, fx) - ox;
+    const int ly = cell_of(r[1], units, d, fy) - oy;
+    const int lz = cell_of(r[2], units, d, fz) - oz;
+    if ((unsigned)lx >= (unsigned)P.kg.TX || (unsigned)ly >= (unsigned)P.kg.TY || (unsigned)lz >= (unsigned)P.kg.TZ)
+      continue;  // cannot happen for records of this tile; guards the LDS bounds
+    const int at = lx * sx + ly * sy + lz;
+    const double g000 = (double)g[at], g001 = (double)g[at + 1];
+    const double g100 = (double)g[at + sx], g101 = (double)g[at + sx + 1];
+    const double g010 = (double)g[at + sy], g011 = (double)g[at + sy + 1];
+    const double g110 = (double)g[at + sx + sy], g111 = (double)g[at + sx + sy + 1];
+    // corners (bx, by, bz); weights (1-f or f) per axis
+    const double ex = 1.0 - fx, ey = 1.0 - fy, ez = 1.0 - fz;
+    // interpolate along z first
+    const double a00 = ez * g000 + fz * g001, a10 = ez * g100 + fz * g101;
+    const double a01 = ez * g010 + fz * g011, a11 = ez * g110 + fz * g111;
+    const double dgx = ey * (a10 - a00) + fy * (a11 - a01);
+    const double dgy = ex * (a01 - a00) + fx * (a11 - a10);
+    const double b00 = ex * ey, b10 = fx * ey, b01 = ex * fy, b11 = fx * fy;
+    const double dgz = b00 * (g001 - g000) + b10 * (g101 - g100) + b01 * (g011 - g010) + b11 * (g111 - g110);
+    const double gm = b00 * a00 + b10 * a10 + b01 * a01 + b11 * a11;
+    const double m = NM ? (double)r[3] : P.mass;
+    const double s = m * units;
+    const uint32_t idx = __float_as_uint(r[RW - 1]);
+    gpos[3 * (int64_t)idx + 0] = (float)(dgx * s);
+    gpos[3 * (int64_t)idx + 1] = (float)(dgy * s);
+    gpos[3 * (int64_t)idx + 2] = (float)(dgz * s);
+    if (gmass) gmass[idx] = (float)gm;
+  }
+}
+
+AdjWeight adj_weight(Plan &p, int nrbins, const double *lambda, double m) {
+  const BinTables &bt = bin_tables(p, nrbins);
+  AdjWeight w;
+  w.q2bin = bt.q2bin.as<int>();
+  w.count = bt.count.as<int>();
+  w.lambda = lambda;
+  w.wtab = p.wtab.as<float>();
+  w.inv_m2 = 1.0 / (m * m);
+  return w;
+}
+
+template <typename T>
+void adjoint_x_impl(Plan &p, const AdjWeight &w) {
+  using C = typename Cx<T>::C;
+  const int B = col_block(p.prec, p.d, p.hz);
+  const int nkb = (int)ceil_div(p.hz, B);
+  const RadixPlan rp = make_radix_plan((int)p.d);
+  const C *tw = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
+  GENPK_REQUIRE((int64_t)p.d * B <= (int64_t)AdjCfg<T>::MAXE * ADJ_THREADS, "adjoint x pass: column block too large");
+  Stage st(p, "adjoint_x");
+  hipLaunchKernelGGL(k_adjoint_x<T>, dim3((unsigned)(p.d * nkb)), dim3(ADJ_THREADS), (size_t)p.d * (B + 1) * sizeof(C),
+                     p.stream, p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, rp, tw, w);
+  GENPK_LAUNCH_CHECK();
+}
+
+template <typename T, int NM>
+void gather_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass, float *grad_pos, float *grad_mass) {
+  const TileGeom &tg = p.tg;
+  GatherParams P;
+  P.kg = kg;
+  P.HY = tg.HY;
+  P.HZ = tg.HZ;
+  P.cells = tg.cells;
+  P.ks = o.ks;
+  P.rowlen = p.rowlen;
+  P.mass = mass;
+  P.fhz = FastDiv((uint32_t)tg.HZ);
+  P.fhy = FastDiv((uint32_t)tg.HY);
+  hipLaunchKernelGGL((k_gather_tiles<T, NM>), dim3((unsigned)o.max_items), dim3(GATHER_THREADS),
+                     (size_t)tg.cells * sizeof(T), p.stream, p.field.as<const T>(), o.rec, o.tile_start, o.item_start,
+                     P, grad_pos, grad_mass);
+  GENPK_LAUNCH_CHECK();
 }
 
 }  // namespace
 
-void adjoint_weight(Plan &p, int nrbins, const double *lambda, double m) {
-  GENPK_REQUIRE(p.fstate == FieldState::Spectrum, "adjoint needs the forward spectrum in the plan");
+void adjoint_density(Plan &p, int nrbins, const double *lambda, double m) {
   GENPK_REQUIRE(m != 0, "total mass must be nonzero");
-  const BinTables &bt = bin_tables(p, nrbins);
-  const int64_t total = p.d * p.d * p.hz;
-  const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(total, 256), 65536);
-  {
-  Stage st(p, "adjoint_weight");
-  if (p.prec == 4)
-    hipLaunchKernelGGL(k_weight<float>, dim3(grid), dim3(256), 0, p.stream, p.field.as<float2>(), p.d, p.hz,
-                       bt.q2bin.as<int>(), bt.count.as<int>(), lambda, p.wtab.as<float>(), 1.0 / (m * m));
-  else
-    hipLaunchKernelGGL(k_weight<double>, dim3(grid), dim3(256), 0, p.stream, p.field.as<double2>(), p.d, p.hz,
-                       bt.q2bin.as<int>(), bt.count.as<int>(), lambda, p.wtab.as<float>(), 1.0 / (m * m));
-  GENPK_LAUNCH_CHECK();
+  GENPK_REQUIRE(lambda != nullptr, "null lambda");
+  const AdjWeight w = adj_weight(p, nrbins, lambda, m);
+  if (!p.tiles_live && p.fstate == FieldState::Spectrum) {
+    // kept spectrum: weight every mode, then the inverse passes
+    const int64_t total = p.d * p.d * p.hz;
+    const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(total, 256), 65536);
+    {
+      Stage st(p, "adjoint_weight");
+      if (p.prec == 4)
+        hipLaunchKernelGGL(k_weight<float>, dim3(grid), dim3(256), 0, p.stream, p.field.as<float2>(), p.d, p.hz, w);
+      else
+        hipLaunchKernelGGL(k_weight<double>, dim3(grid), dim3(256), 0, p.stream, p.field.as<double2>(), p.d, p.hz, w);
+      GENPK_LAUNCH_CHECK();
+    }
+    c2r(p);
+    return;
   }
-  c2r(p);
+  if (p.tiles_live || p.fstate == FieldState::Real) {
+    // forward density (deposit or imported): forward z and y passes first
+    r2c_zpass(p);
+    c2c_ypass(p, -1);
+    p.fstate = FieldState::YDone;
+  }
+  GENPK_REQUIRE(p.fstate == FieldState::YDone,
+                "adjoint needs the forward spectrum, the forward density or the state left by the fused power");
+  if (p.prec == 4) adjoint_x_impl<float>(p, w); else adjoint_x_impl<double>(p, w);
+  c2c_ypass(p, +1);
+  c2r_zpass(p);
+  p.fstate = FieldState::Real;
 }
 
 void adjoint_gather(Plan &p, double boxsize, int64_t n, const float *pos, const float *masses, double mass,
                     float *grad_pos, float *grad_mass) {
-  GENPK_REQUIRE(p.fstate == FieldState::Real, "gather needs dL/drho in the plan (run the adjoint first)");
-  GENPK_REQUIRE(boxsize > 0, "boxsize must be positive");
+  GENPK_REQUIRE(p.fstate == FieldState::Real && !p.tiles_live,
+                "gather needs dL/drho in the plan (run the adjoint first)");
+  GENPK_REQUIRE(boxsize > 0 && std::isfinite(boxsize), "boxsize must be positive and finite");
+  GENPK_REQUIRE(n >= 0, "negative particle count");
   if (n == 0) return;
   GENPK_REQUIRE(pos && grad_pos, "null positions or gradient output");
-  const double units = (double)p.d / boxsize;
-  const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(n, 256), 1 << 20);
-  Stage st(p, "adjoint_gather");
-  if (p.prec == 4) {
-    if (masses)
-      hipLaunchKernelGGL((k_gather<float, true>), dim3(grid), dim3(256), 0, p.stream, p.field.as<float>(), p.d,
-                         p.rowlen, units, n, pos, masses, mass, grad_pos, grad_mass);
-    else
-      hipLaunchKernelGGL((k_gather<float, false>), dim3(grid), dim3(256), 0, p.stream, p.field.as<float>(), p.d,
-                         p.rowlen, units, n, pos, masses, mass, grad_pos, grad_mass);
-  } else {
-    if (masses)
-      hipLaunchKernelGGL((k_gather<double, true>), dim3(grid), dim3(256), 0, p.stream, p.field.as<double>(), p.d,
-                         p.rowlen, units, n, pos, masses, mass, grad_pos, grad_mass);
-    else
-      hipLaunchKernelGGL((k_gather<double, false>), dim3(grid), dim3(256), 0, p.stream, p.field.as<double>(), p.d,
-                         p.rowlen, units, n, pos, masses, mass, grad_pos, grad_mass);
-  }
-  GENPK_LAUNCH_CHECK();
+  const int64_t cap = (int64_t)1 << 31;
+  const RetainedOrder &r = p.order;
+  const bool reuse = r.valid && r.pos == pos && r.masses == masses && r.n == n && r.boxsize == boxsize &&
+                     r.to.has_index;
+  const KeyGeom kg = key_geom(p, boxsize);
+  int64_t done = 0;
+  do {
+    const int64_t m = std::min(n - done, cap);
+    const float *mp = masses ? masses + done : nullptr;
+    TileOrder o;
+    if (reuse) {
+      o = r.to;
+    } else {
+      o = sort_by_tile(p, kg, m, pos + 3 * done, mp, true, false);
+      p.order.valid = n <= cap;
+      p.order.pos = pos;
+      p.order.masses = masses;
+      p.order.n = n;
+      p.order.boxsize = boxsize;
+      p.order.to = o;
+    }
+    float *gp = grad_pos + 3 * done;
+    float *gmp = grad_mass ? grad_mass + done : nullptr;
+    Stage st(p, "adjoint_gather");
+    if (p.prec == 4) {
+      if (masses) gather_impl<float, 1>(p, kg, o, mass, gp, gmp); else gather_impl<float, 0>(p, kg, o, mass, gp, gmp);
+    } else {
+      if (masses) gather_impl<double, 1>(p, kg, o, mass, gp, gmp); else gather_impl<double, 0>(p, kg, o, mass, gp, gmp);
+    }
+    done += m;
+  } while (done < n);
 }
 
 }  // namespace genpk

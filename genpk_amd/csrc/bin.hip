@@ -474,8 +474,9 @@ void bin_spectra(Plan &a, Plan *b, int nrbins, bool fused_xpass, double *pow_aa,
     } else {
       if (b) launch_fused<double, 2>(a, b, bt, B, false, part, G); else launch_fused<double, 1>(a, b, bt, B, false, part, G);
     }
-    a.fstate = FieldState::None;
-    if (b) b->fstate = FieldState::None;
+    // the field keeps the forward state after the z and y passes (an adjoint can start from it)
+    a.fstate = FieldState::YDone;
+    if (b) b->fstate = FieldState::YDone;
   } else {
     GENPK_REQUIRE(a.fstate == FieldState::Spectrum && (!b || b->fstate == FieldState::Spectrum),
                   "binning needs transformed fields");

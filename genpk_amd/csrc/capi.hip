@@ -73,6 +73,12 @@ Plan::Plan(int64_t dims, int precision, hipStream_t s) {
   if (fftable) fft_tables_init(*this);
 }
 
+// The field holds no density (None), a density (Real), or the forward state
+// left by the fused power (YDone), which counts as no density.
+static bool density_state(const Plan &p) {
+  return p.fstate == FieldState::None || p.fstate == FieldState::Real || p.fstate == FieldState::YDone;
+}
+
 static bool fft_ok(const Plan &p) {
   return (p.d % 2 == 0) && p.d >= 2 && fft_length_ok(p.d) && fft_length_ok(p.d / 2);
 }
@@ -190,9 +196,18 @@ int genpk_deposit(genpk_plan *plan, double boxsize, int64_t n, const float *pos,
   return guarded([&] {
     GENPK_REQUIRE(plan && plan->p, "null plan");
     Plan &p = *plan->p;
-    GENPK_REQUIRE(p.fstate == FieldState::None || p.fstate == FieldState::Real,
-                  "deposit after a transform: clear the plan first");
+    GENPK_REQUIRE(density_state(p), "deposit after a transform: clear the plan first");
     deposit(p, boxsize, n, pos, masses, mass);
+  });
+}
+
+int genpk_deposit_retain(genpk_plan *plan, double boxsize, int64_t n, const float *pos, const float *masses,
+                         double mass) {
+  return guarded([&] {
+    GENPK_REQUIRE(plan && plan->p, "null plan");
+    Plan &p = *plan->p;
+    GENPK_REQUIRE(density_state(p), "deposit after a transform: clear the plan first");
+    deposit(p, boxsize, n, pos, masses, mass, true);
   });
 }
 
@@ -200,7 +215,7 @@ int genpk_export_density(genpk_plan *plan, void *out, int64_t out_rowlen, int ac
   return guarded([&] {
     GENPK_REQUIRE(plan && plan->p && out, "null argument");
     Plan &p = *plan->p;
-    GENPK_REQUIRE(p.fstate == FieldState::None || p.fstate == FieldState::Real, "the plan holds no density");
+    GENPK_REQUIRE(density_state(p), "the plan holds no density");
     GENPK_REQUIRE(out_rowlen >= p.d, "out_rowlen shorter than dims");
     if (p.fstate == FieldState::Real) {
       // imported density plus deposits: assemble in the plan's grid first
@@ -227,12 +242,23 @@ int genpk_import_density(genpk_plan *plan, const void *grid) {
   });
 }
 
+int genpk_import_spectrum(genpk_plan *plan, const void *spectrum) {
+  return guarded([&] {
+    GENPK_REQUIRE(plan && plan->p && spectrum, "null argument");
+    Plan &p = *plan->p;
+    if (spectrum != p.field.ptr)
+      GENPK_HIP(hipMemcpyAsync(p.field.ptr, spectrum, p.field_elems * p.prec, hipMemcpyDeviceToDevice, p.stream));
+    p.fstate = FieldState::Spectrum;
+    p.tiles_live = false;
+  });
+}
+
 int genpk_r2c(genpk_plan *plan) {
   return guarded([&] {
     GENPK_REQUIRE(plan && plan->p, "null plan");
     Plan &p = *plan->p;
     GENPK_REQUIRE(fft_ok(p), "the FFT needs an even dims with dims/2 = 2^a 3^b 5^c");
-    GENPK_REQUIRE(p.fstate == FieldState::None || p.fstate == FieldState::Real, "the plan holds no density");
+    GENPK_REQUIRE(density_state(p), "the plan holds no density");
     r2c(p);
   });
 }
@@ -261,8 +287,8 @@ int genpk_power(genpk_plan *a, genpk_plan *b, int nrbins, double *power_aa, doub
     Plan &pa = *a->p;
     Plan *pb = b ? b->p.get() : nullptr;
     GENPK_REQUIRE(fft_ok(pa), "the FFT needs an even dims with dims/2 = 2^a 3^b 5^c");
-    GENPK_REQUIRE(pa.fstate == FieldState::None || pa.fstate == FieldState::Real, "plan a holds no density");
-    if (pb) GENPK_REQUIRE(pb->fstate == FieldState::None || pb->fstate == FieldState::Real, "plan b holds no density");
+    GENPK_REQUIRE(density_state(pa), "plan a holds no density");
+    if (pb) GENPK_REQUIRE(density_state(*pb), "plan b holds no density");
     (void)bin_tables(pa, nrbins);
     for (Plan *q : {&pa, pb}) {
       if (!q) continue;
@@ -278,7 +304,7 @@ int genpk_adjoint_density(genpk_plan *plan, int nrbins, const double *lambda, do
   return guarded([&] {
     GENPK_REQUIRE(plan && plan->p && lambda, "null argument");
     GENPK_REQUIRE(fft_ok(*plan->p), "the FFT needs an even dims with dims/2 = 2^a 3^b 5^c");
-    adjoint_weight(*plan->p, nrbins, lambda, m);
+    adjoint_density(*plan->p, nrbins, lambda, m);
   });
 }
 

@@ -82,6 +82,27 @@ struct BinTables {
 
 enum class FieldState { None, Real, Spectrum, YDone };
 
+// Particles partitioned by tile (plan scratch; see sort_by_tile): records
+// of rw floats (x, y, z[, mass][, index]), tile starts, work items.
+struct TileOrder {
+  const float *rec = nullptr;
+  int rw = 0;
+  bool has_mass = false, has_index = false;
+  const uint32_t *tile_start = nullptr, *item_start = nullptr, *part_start = nullptr;
+  const uint32_t *nheavy = nullptr, *heavy = nullptr;
+  int64_t ks = 0, n = 0, max_items = 0;
+};
+
+// Tile order kept from the last deposit made with retain = true, for the
+// adjoint gather of the same particle arrays.
+struct RetainedOrder {
+  bool valid = false;
+  const float *pos = nullptr, *masses = nullptr;
+  int64_t n = 0;
+  double boxsize = 0;
+  TileOrder to;
+};
+
 // Stage timing: hipEvents recorded on the plan's stream around each stage
 // while enabled; read back (and cleared) by genpk_timing_read().
 struct StageTimer {
@@ -126,6 +147,7 @@ struct Plan {
   // sort / deposit scratch
   DevBuf recA, recB, table1, table2, seginfo, tile_start, item_start, part_start, heavy, partials;
   DevBuf scan_tmp;
+  RetainedOrder order;
 
   // window factors: float(onedinvwindow(KVAL(k), d)), k < d
   DevBuf wtab;
@@ -162,7 +184,8 @@ struct FoldGeom;
 FoldGeom fold_geom(const Plan &p);
 
 // deposit.hip
-void deposit(Plan &p, double boxsize, int64_t n, const float *pos, const float *masses, double mass);
+void deposit(Plan &p, double boxsize, int64_t n, const float *pos, const float *masses, double mass,
+             bool retain = false);
 void fold_tiles(Plan &p, void *out, int64_t out_rowlen, bool accumulate);
 void tiles_to_field(Plan &p);
 // scan.hip
@@ -180,7 +203,7 @@ BinTables &bin_tables(Plan &p, int nrbins);
 void bin_spectra(Plan &a, Plan *b, int nrbins, bool fused_xpass, double *pow_aa, double *pow_bb,
                  double *pow_ab, int *count, double *keffs, double ma, double mb);
 // adjoint.hip
-void adjoint_weight(Plan &p, int nrbins, const double *lambda, double m);
+void adjoint_density(Plan &p, int nrbins, const double *lambda, double m);
 void adjoint_gather(Plan &p, double boxsize, int64_t n, const float *pos, const float *masses, double mass,
                     float *grad_pos, float *grad_mass);
 

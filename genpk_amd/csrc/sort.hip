@@ -1,0 +1,500 @@
+// Partition of particles by deposit tile on gfx950: an MSD radix partition
+// of one or two passes over the tile key (upsweep histogram in LDS ->
+// offset scan -> downsweep that ranks records with wavefront ballots,
+// stages them in LDS by digit and writes contiguous runs), then the work
+// items of the tile kernels.  Global atomics are avoided: on this chip they
+// run at ~27 Gop/s.  The partition is stable, so the records of one tile
+// keep their input order.
+#include "tilesort.h"
+
+namespace genpk {
+
+namespace {
+
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_WAVES = SORT_THREADS / 64;
+constexpr int SORT_PER_LANE = 16;                                   // iterations per wave per sub-chunk
+constexpr int SORT_SUB = SORT_THREADS * SORT_PER_LANE;              // 4096 records per sub-chunk
+constexpr int SORT_CHUNK = 8 * SORT_SUB;                            // 32768 records per workgroup
+
+// Source of records: either the caller's arrays (positions [n][3], optional
+// masses [n]; the index payload is the record's position in [0, n)) or a
+// packed record buffer of RW floats per record.
+template <int NM, int NI>
+struct Src {
+  static constexpr int RW = 3 + NM + NI;
+  const float *pos;     // raw positions (stride 3) or packed records (stride RW)
+  const float *masses;  // raw masses when reading the caller's arrays
+  bool raw;
+  __device__ __forceinline__ void load_pos(int64_t i, float *r) const {
+    const int64_t st = raw ? 3 : RW;
+    r[0] = pos[st * i + 0];
+    r[1] = pos[st * i + 1];
+    r[2] = pos[st * i + 2];
+  }
+  __device__ __forceinline__ void load(int64_t i, float *r) const {
+    if (raw) {
+      r[0] = pos[3 * i + 0];
+      r[1] = pos[3 * i + 1];
+      r[2] = pos[3 * i + 2];
+      if (NM) r[3] = masses[i];
+      if (NI) r[3 + NM] = __uint_as_float((uint32_t)i);
+    } else {
+#pragma unroll
+      for (int k = 0; k < RW; k++) r[k] = pos[RW * i + k];
+    }
+  }
+};
+
+// Chunk range of workgroup g.  Pass 1 chunks tile [0, n); pass 2 chunks
+// tile each high-digit segment separately.
+struct ChunkMap {
+  int64_t n;
+  int nseg;                    // 0: plain chunks of [0, n)
+  const uint32_t *seg_start;   // [nseg+1] record offsets of the segments
+  const uint32_t *chunk_start; // [nseg+1] first chunk of each segment
+  __device__ __forceinline__ bool range(int64_t g, int64_t &b, int64_t &e) const {
+    if (nseg == 0) {
+      b = g * SORT_CHUNK;
+      e = min(n, b + SORT_CHUNK);
+      return b < e;
+    }
+    if (g >= (int64_t)chunk_start[nseg]) return false;
+    int lo = 0, hi = nseg;  // find s with chunk_start[s] <= g < chunk_start[s+1]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if ((int64_t)chunk_start[mid] <= g) lo = mid; else hi = mid;
+    }
+    const int s = lo;
+    b = (int64_t)seg_start[s] + (g - (int64_t)chunk_start[s]) * SORT_CHUNK;
+    e = min((int64_t)seg_start[s + 1], b + SORT_CHUNK);
+    return b < e;
+  }
+};
+
+struct DigitSel {
+  int shift;
+  uint32_t mask;
+  __device__ __forceinline__ uint32_t of(uint32_t key) const { return (key >> shift) & mask; }
+};
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const int lane = threadIdx.x & 63;
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// counts[dg * sd + g * sg] = number of records of chunk g with digit dg
+template <int NM, int NI>
+__global__ void __launch_bounds__(SORT_THREADS) k_upsweep(Src<NM, NI> src, ChunkMap cm, KeyGeom kg, DigitSel ds,
+                                                          int nb, uint32_t *__restrict__ counts, int64_t sd,
+                                                          int64_t sg) {
+  constexpr int UNR = 8;
+  extern __shared__ uint32_t hist[];
+  int64_t b, e;
+  if (!cm.range(blockIdx.x, b, e)) return;
+  for (int i = threadIdx.x; i < nb; i += SORT_THREADS) hist[i] = 0;
+  __syncthreads();
+  for (int64_t base = b; base < e; base += SORT_THREADS * UNR) {
+    float r[UNR][3];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const int64_t i = base + u * SORT_THREADS + threadIdx.x;
+      if (i < e) src.load_pos(i, r[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const int64_t i = base + u * SORT_THREADS + threadIdx.x;
+      const bool valid = i < e;
+      const uint32_t dg = valid ? ds.of(tile_key(r[u], kg)) : 0u;
+      // a wavefront whose records share one digit adds them with one atomic
+      const uint64_t vmask = __ballot(valid);
+      const uint32_t d0 = __shfl(dg, (int)__builtin_ctzll(vmask | (1ull << 63)), 64);
+      if (__ballot(valid && dg == d0) == vmask) {
+        if (valid && (vmask & lanemask_lt()) == 0) atomicAdd(&hist[d0], (uint32_t)__popcll(vmask));
+      } else if (valid) {
+        atomicAdd(&hist[dg], 1u);
+      }
+    }  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb; i += SORT_THREADS)
+    counts[(int64_t)i * sd + (int64_t)blockIdx.x * sg] = hist[i];
+}
+
+// Stable partition of each chunk by digit into dst at offsets[dg*sd + g*sg].
+// Per sub-chunk of SORT_SUB records (each wave owns 1024 consecutive ones):
+// wavefront ballots give every record its rank among equal digits, LDS
+// prefix sums turn ranks into positions of a digit-sorted staging copy, and
+// the staging copy is written out as contiguous runs.  The next sub-chunk's
+// records are loaded while the current one is written out.
+template <int NM, int NI>
+__global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<NM, NI> src, float *__restrict__ dst, ChunkMap cm,
+                                                            KeyGeom kg, DigitSel ds, int nb, int dbits,
+                                                            const uint32_t *__restrict__ offsets, int64_t sd,
+                                                            int64_t sg) {
+  constexpr int RW = Src<NM, NI>::RW;
+  extern __shared__ uint32_t sm[];
+  uint32_t *whist = sm;                         // [SORT_WAVES][nb]
+  uint32_t *lstart = whist + SORT_WAVES * nb;   // [nb]
+  uint32_t *gbase = lstart + nb;                // [nb]
+  uint32_t *subcnt = gbase + nb;                // [nb]
+  uint16_t *sdig = reinterpret_cast<uint16_t *>(subcnt + nb);          // [SORT_SUB]
+  float *stage = reinterpret_cast<float *>(sdig + SORT_SUB);           // [SORT_SUB][RW]
+  __shared__ uint32_t wtot[SORT_WAVES];
+
+  int64_t cb, ce;
+  if (!cm.range(blockIdx.x, cb, ce)) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = threadIdx.x; i < nb; i += SORT_THREADS)
+    gbase[i] = offsets[(int64_t)i * sd + (int64_t)blockIdx.x * sg];
+  for (int i = threadIdx.x; i < SORT_WAVES * nb; i += SORT_THREADS) whist[i] = 0;
+
+  float rec[SORT_PER_LANE][RW];
+  const int wbase = wave * (SORT_PER_LANE * 64) + lane;
+#pragma unroll
+  for (int it = 0; it < SORT_PER_LANE; it++) {
+    const int64_t i = cb + wbase + it * 64;
+    if (i < ce) src.load(i, rec[it]);
+  }
+  __syncthreads();
+
+  for (int64_t s0 = cb; s0 < ce; s0 += SORT_SUB) {
+    const int nsub = (int)min((int64_t)SORT_SUB, ce - s0);
+    uint32_t dig[SORT_PER_LANE];
+    uint32_t rnk[SORT_PER_LANE];
+#pragma unroll
+    for (int it = 0; it < SORT_PER_LANE; it++) {
+      const int loc = wbase + it * 64;
+      const bool valid = loc < nsub;
+      const uint32_t dg = valid ? ds.of(tile_key(rec[it], kg)) : 0u;
+      uint64_t peers = __ballot(valid);
+      // wavefronts whose records share one digit skip the per-bit match
+      const uint32_t d0 = __shfl(dg, (int)__builtin_ctzll(peers | (1ull << 63)), 64);
+      if (__ballot(valid && dg == d0) != peers) {
+        for (int bit = 0; bit < dbits; bit++) {
+          const bool set = (dg >> bit) & 1u;
+          const uint64_t bb = __ballot(set);
+          peers &= set ? bb : ~bb;
+        }
+      }
+      dig[it] = dg;
+      rnk[it] = 0;
+      if (valid) {
+        const uint32_t before = whist[wave * nb + dg];
+        rnk[it] = before + (uint32_t)__popcll(peers & lanemask_lt());
+        // every lane has read the running count before the group leader updates it
+        __builtin_amdgcn_wave_barrier();
+        if ((peers & lanemask_lt()) == 0) whist[wave * nb + dg] = before + (uint32_t)__popcll(peers);
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // per-digit exclusive prefix over waves and sub-chunk totals
+    for (int dgi = threadIdx.x; dgi < nb; dgi += SORT_THREADS) {
+      uint32_t tot = 0;
+#pragma unroll
+      for (int w = 0; w < SORT_WAVES; w++) {
+        const uint32_t t = whist[w * nb + dgi];
+        whist[w * nb + dgi] = tot;
+        tot += t;
+      }
+      subcnt[dgi] = tot;
+    }
+    __syncthreads();
+    // exclusive scan of subcnt -> lstart (nb <= 2048)
+    {
+      const int per = (nb + SORT_THREADS - 1) / SORT_THREADS;
+      const int first = threadIdx.x * per;
+      uint32_t tsum = 0;
+      for (int i = 0; i < per; i++)
+        if (first + i < nb) tsum += subcnt[first + i];
+      uint32_t incl = tsum;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+      }
+      if (lane == 63) wtot[wave] = incl;
+      __syncthreads();
+      uint32_t run = incl - tsum;
+      for (int w = 0; w < wave; w++) run += wtot[w];
+      for (int i = 0; i < per; i++)
+        if (first + i < nb) {
+          lstart[first + i] = run;
+          run += subcnt[first + i];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < SORT_PER_LANE; it++) {
+      const int loc = wbase + it * 64;
+      if (loc < nsub) {
+        const uint32_t dg = dig[it];
+        const uint32_t p = lstart[dg] + whist[wave * nb + dg] + rnk[it];
+#pragma unroll
+        for (int k = 0; k < RW; k++) stage[p * RW + k] = rec[it][k];
+        sdig[p] = (uint16_t)dg;
+      }
+    }
+    __syncthreads();
+    // prefetch the next sub-chunk while this one is written out
+    {
+      const int64_t n0 = s0 + SORT_SUB;
+#pragma unroll
+      for (int it = 0; it < SORT_PER_LANE; it++) {
+        const int64_t i = n0 + wbase + it * 64;
+        if (i < ce) src.load(i, rec[it]);
+      }
+    }
+    for (int p = threadIdx.x; p < nsub; p += SORT_THREADS) {
+      const uint32_t dg = sdig[p];
+      const int64_t dest = (int64_t)gbase[dg] + (p - (int64_t)lstart[dg]);
+#pragma unroll
+      for (int k = 0; k < RW; k++) dst[dest * RW + k] = stage[p * RW + k];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += SORT_THREADS) gbase[i] += subcnt[i];
+    for (int i = threadIdx.x; i < SORT_WAVES * nb; i += SORT_THREADS) whist[i] = 0;
+    __syncthreads();
+  }
+}
+
+// Pass-2 segment bookkeeping from the scanned pass-1 table.
+__global__ void k_seginfo(const uint32_t *__restrict__ t1scan, int64_t g1, int nseg, int64_t n,
+                          uint32_t *__restrict__ seg_start, uint32_t *__restrict__ nchunks) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > nseg) return;
+  if (s == nseg) {
+    seg_start[s] = (uint32_t)n;
+    nchunks[s] = 0;
+    return;
+  }
+  const uint32_t st = t1scan[(int64_t)s * g1];
+  const uint32_t en = (s + 1 < nseg) ? t1scan[(int64_t)(s + 1) * g1] : (uint32_t)n;
+  seg_start[s] = st;
+  nchunks[s] = (en - st + SORT_CHUNK - 1) / SORT_CHUNK;
+}
+
+// tot[hi*nlo+lo] = sum over the chunks of segment hi of cnt2[g*nlo+lo]
+__global__ void k_seg_total(const uint32_t *__restrict__ cnt2, const uint32_t *__restrict__ chunk_start, int nhi,
+                            int nlo, uint32_t *__restrict__ tot) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nhi * nlo) return;
+  const int hi = (int)(t / nlo), lo = (int)(t % nlo);
+  uint32_t s = 0;
+  for (uint32_t g = chunk_start[hi]; g < chunk_start[hi + 1]; g++) s += cnt2[(int64_t)g * nlo + lo];
+  tot[t] = s;
+}
+
+// cnt2 -> per-chunk output offsets, starting at the tile start
+__global__ void k_seg_offsets(uint32_t *__restrict__ cnt2, const uint32_t *__restrict__ chunk_start, int nhi, int nlo,
+                              const uint32_t *__restrict__ tile_start) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nhi * nlo) return;
+  const int hi = (int)(t / nlo), lo = (int)(t % nlo);
+  uint32_t run = tile_start[t];
+  for (uint32_t g = chunk_start[hi]; g < chunk_start[hi + 1]; g++) {
+    const uint32_t c = cnt2[(int64_t)g * nlo + lo];
+    cnt2[(int64_t)g * nlo + lo] = run;
+    run += c;
+  }
+}
+
+// Single-pass tile starts: tile_start[k] = t1scan[k*g1] (chunk 0), tile_start[ks] = n
+__global__ void k_tile_start_1pass(const uint32_t *__restrict__ t1scan, int64_t g1, int ks, int64_t n,
+                                   uint32_t *__restrict__ tile_start) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > ks) return;
+  tile_start[k] = (k == ks) ? (uint32_t)n : t1scan[(int64_t)k * g1];
+}
+
+__global__ void k_set_last(uint32_t *__restrict__ a, int64_t at, uint32_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) a[at] = v;
+}
+
+// Work items per tile.  store mode: every real tile gets at least one item
+// (so untouched tiles are written as zeros); add mode: empty tiles are skipped.
+__global__ void k_items(const uint32_t *__restrict__ tile_start, int64_t ks, int64_t kreal, int store,
+                        uint32_t *__restrict__ mcnt, uint32_t *__restrict__ pcnt, uint32_t *__restrict__ nheavy) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > ks) return;
+  if (k == ks) {
+    mcnt[k] = 0;
+    pcnt[k] = 0;
+    return;
+  }
+  const uint32_t c = tile_start[k + 1] - tile_start[k];
+  uint32_t m = 0;
+  if (k < kreal) m = c == 0 ? (store ? 1u : 0u) : (c + DEPOSIT_ITEM_MAX - 1) / DEPOSIT_ITEM_MAX;
+  mcnt[k] = m;
+  pcnt[k] = m > 1 ? m : 0;
+  if (k == 0) *nheavy = 0;
+}
+
+__global__ void k_heavy_list(const uint32_t *__restrict__ pcnt, int64_t ks, uint32_t *__restrict__ nheavy,
+                             uint32_t *__restrict__ heavy) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ks) return;
+  if (pcnt[k] > 1) heavy[atomicAdd(nheavy, 1u)] = (uint32_t)k;
+}
+
+size_t downsweep_lds(int nb, int rw) {
+  return (size_t)(SORT_WAVES * nb + 3 * nb) * 4 + (size_t)SORT_SUB * 2 + (size_t)SORT_SUB * rw * 4;
+}
+
+template <int NM, int NI>
+TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, const float *masses, bool every_tile) {
+  constexpr int RW = Src<NM, NI>::RW;
+  hipStream_t s = p.stream;
+  const TileGeom &tg = p.tg;
+
+  float *A = p.recA.as<float>();
+  float *B = p.recB.as<float>();
+  const int64_t g1 = ceil_div(n, SORT_CHUNK);
+  Src<NM, NI> raw{pos, masses, true};
+  const float *sorted = nullptr;
+  uint32_t *tile_start = nullptr;
+  int64_t ks = 0;
+
+  if (tg.passes == 1) {
+    const int nb = 1 << tg.hi_bits;
+    ks = nb;
+    uint32_t *t1 = static_cast<uint32_t *>(p.table1.ensure((size_t)nb * g1 * 4));
+    DigitSel dsel{0, (uint32_t)(nb - 1)};
+    ChunkMap cm{n, 0, nullptr, nullptr};
+    {
+      Stage st(p, "sort_upsweep1");
+      hipLaunchKernelGGL((k_upsweep<NM, NI>), dim3((unsigned)g1), dim3(SORT_THREADS), nb * 4, s, raw, cm, kg, dsel, nb, t1,
+                         g1, (int64_t)1);
+      GENPK_LAUNCH_CHECK();
+    }
+    exclusive_scan_u32(t1, t1, (int64_t)nb * g1, p.scan_tmp, s);
+    {
+      Stage st(p, "sort_downsweep1");
+      hipLaunchKernelGGL((k_downsweep<NM, NI>), dim3((unsigned)g1), dim3(SORT_THREADS), downsweep_lds(nb, RW), s, raw, A,
+                         cm, kg, dsel, nb, tg.hi_bits, t1, g1, (int64_t)1);
+      GENPK_LAUNCH_CHECK();
+    }
+    tile_start = static_cast<uint32_t *>(p.tile_start.ensure((size_t)(ks + 1) * 4));
+    hipLaunchKernelGGL(k_tile_start_1pass, dim3((unsigned)ceil_div(ks + 1, 256)), dim3(256), 0, s, t1, g1, (int)ks,
+                       n, tile_start);
+    GENPK_LAUNCH_CHECK();
+    sorted = A;
+  } else {
+    const int nhi = 1 << tg.hi_bits, nlo = 1 << tg.lo_bits;
+    ks = (int64_t)nhi * nlo;
+    // pass 1: high digit over [0, n)
+    uint32_t *t1 = static_cast<uint32_t *>(p.table1.ensure((size_t)nhi * g1 * 4));
+    DigitSel d1{tg.lo_bits, (uint32_t)(nhi - 1)};
+    ChunkMap cm1{n, 0, nullptr, nullptr};
+    {
+      Stage st(p, "sort_upsweep1");
+      hipLaunchKernelGGL((k_upsweep<NM, NI>), dim3((unsigned)g1), dim3(SORT_THREADS), nhi * 4, s, raw, cm1, kg, d1, nhi,
+                         t1, g1, (int64_t)1);
+      GENPK_LAUNCH_CHECK();
+    }
+    exclusive_scan_u32(t1, t1, (int64_t)nhi * g1, p.scan_tmp, s);
+    {
+      Stage st(p, "sort_downsweep1");
+      hipLaunchKernelGGL((k_downsweep<NM, NI>), dim3((unsigned)g1), dim3(SORT_THREADS), downsweep_lds(nhi, RW), s, raw, A,
+                         cm1, kg, d1, nhi, tg.hi_bits, t1, g1, (int64_t)1);
+      GENPK_LAUNCH_CHECK();
+    }
+    // segments of pass 2
+    uint32_t *seg = static_cast<uint32_t *>(p.seginfo.ensure((size_t)(nhi + 1) * 3 * 4));
+    uint32_t *seg_start = seg, *nchunks = seg + (nhi + 1), *chunk_start = seg + 2 * (nhi + 1);
+    hipLaunchKernelGGL(k_seginfo, dim3((unsigned)ceil_div(nhi + 1, 256)), dim3(256), 0, s, t1, g1, nhi, n, seg_start,
+                       nchunks);
+    GENPK_LAUNCH_CHECK();
+    exclusive_scan_u32(nchunks, chunk_start, nhi + 1, p.scan_tmp, s);
+    const int64_t g2 = g1 + nhi;  // upper bound of pass-2 chunks
+    uint32_t *t2 = static_cast<uint32_t *>(p.table2.ensure((size_t)g2 * nlo * 4 + (size_t)(ks + 1) * 4 * 2));
+    uint32_t *tot = t2 + g2 * nlo;
+    DigitSel d2{0, (uint32_t)(nlo - 1)};
+    ChunkMap cm2{n, nhi, seg_start, chunk_start};
+    Src<NM, NI> pa{A, nullptr, false};
+    {
+      Stage st(p, "sort_upsweep2");
+      hipLaunchKernelGGL((k_upsweep<NM, NI>), dim3((unsigned)g2), dim3(SORT_THREADS), nlo * 4, s, pa, cm2, kg, d2, nlo, t2,
+                         (int64_t)1, (int64_t)nlo);
+      GENPK_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_seg_total, dim3((unsigned)ceil_div(ks, 256)), dim3(256), 0, s, t2, chunk_start, nhi, nlo,
+                       tot);
+    GENPK_LAUNCH_CHECK();
+    tile_start = static_cast<uint32_t *>(p.tile_start.ensure((size_t)(ks + 1) * 4));
+    hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, s, tot, ks, 0u);
+    GENPK_LAUNCH_CHECK();
+    exclusive_scan_u32(tot, tile_start, ks + 1, p.scan_tmp, s);
+    hipLaunchKernelGGL(k_seg_offsets, dim3((unsigned)ceil_div(ks, 256)), dim3(256), 0, s, t2, chunk_start, nhi, nlo,
+                       tile_start);
+    GENPK_LAUNCH_CHECK();
+    {
+      Stage st(p, "sort_downsweep2");
+      hipLaunchKernelGGL((k_downsweep<NM, NI>), dim3((unsigned)g2), dim3(SORT_THREADS), downsweep_lds(nlo, RW), s, pa, B,
+                         cm2, kg, d2, nlo, tg.lo_bits, t2, (int64_t)1, (int64_t)nlo);
+      GENPK_LAUNCH_CHECK();
+    }
+    sorted = B;
+  }
+
+  // work items
+  const int store = every_tile ? 1 : 0;
+  uint32_t *ib = static_cast<uint32_t *>(p.item_start.ensure((size_t)(ks + 1) * 4 * 4 + 64));
+  uint32_t *mcnt = ib, *item_start = ib + (ks + 1), *pcnt = ib + 2 * (ks + 1), *part_start = ib + 3 * (ks + 1);
+  uint32_t *hv = static_cast<uint32_t *>(p.heavy.ensure((size_t)(ks + 2) * 4));
+  uint32_t *nheavy = hv, *heavy = hv + 1;
+  hipLaunchKernelGGL(k_items, dim3((unsigned)ceil_div(ks + 1, 256)), dim3(256), 0, s, tile_start, ks, tg.K, store,
+                     mcnt, pcnt, nheavy);
+  GENPK_LAUNCH_CHECK();
+  exclusive_scan_u32(mcnt, item_start, ks + 1, p.scan_tmp, s);
+  exclusive_scan_u32(pcnt, part_start, ks + 1, p.scan_tmp, s);
+  hipLaunchKernelGGL(k_heavy_list, dim3((unsigned)ceil_div(ks, 256)), dim3(256), 0, s, pcnt, ks, nheavy, heavy);
+  GENPK_LAUNCH_CHECK();
+
+  const int64_t max_items = tg.K + ceil_div(n, DEPOSIT_ITEM_MAX);
+  TileOrder o;
+  o.rec = sorted;
+  o.rw = RW;
+  o.has_mass = NM != 0;
+  o.has_index = NI != 0;
+  o.tile_start = tile_start;
+  o.item_start = item_start;
+  o.part_start = part_start;
+  o.nheavy = nheavy;
+  o.heavy = heavy;
+  o.ks = ks;
+  o.n = n;
+  o.max_items = max_items;
+  return o;
+}
+}  // namespace
+
+KeyGeom key_geom(const Plan &p, double boxsize) {
+  KeyGeom g;
+  g.units = (double)p.d / boxsize;
+  g.d = (int)p.d;
+  g.TX = p.tg.TX; g.TY = p.tg.TY; g.TZ = p.tg.TZ;
+  g.sx = p.tg.sx; g.sy = p.tg.sy; g.sz = p.tg.sz;
+  g.ntx = p.tg.ntx; g.nty = p.tg.nty; g.ntz = p.tg.ntz;
+  return g;
+}
+
+size_t downsweep_lds(int nb, int rw) {
+  return (size_t)(SORT_WAVES * nb + 3 * nb) * 4 + (size_t)SORT_SUB * 2 + (size_t)SORT_SUB * rw * 4;
+}
+
+TileOrder sort_by_tile(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, const float *masses, bool with_index,
+                       bool every_tile) {
+  GENPK_REQUIRE(n > 0 && n <= ((int64_t)1 << 31), "sort_by_tile: particle count out of range");
+  const int rw = 3 + (masses ? 1 : 0) + (with_index ? 1 : 0);
+  const size_t rec_bytes = (size_t)n * rw * 4;
+  p.recA.ensure(rec_bytes);
+  if (p.tg.passes == 2) p.recB.ensure(rec_bytes);
+  if (masses)
+    return with_index ? sort_impl<1, 1>(p, kg, n, pos, masses, every_tile)
+                      : sort_impl<1, 0>(p, kg, n, pos, masses, every_tile);
+  return with_index ? sort_impl<0, 1>(p, kg, n, pos, nullptr, every_tile)
+                    : sort_impl<0, 0>(p, kg, n, pos, nullptr, every_tile);
+}
+
+}  // namespace genpk

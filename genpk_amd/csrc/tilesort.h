@@ -1,0 +1,77 @@
+// Partition of particles by deposit tile (sort.hip) and the per-particle
+// cell arithmetic shared by the deposit and the adjoint gather.
+#pragma once
+#include "plan.h"
+
+namespace genpk {
+
+struct KeyGeom {
+  double units;
+  int d;
+  int TX, TY, TZ;      // tile edge, powers of two
+  int sx, sy, sz;      // log2 of the tile edges
+  int ntx, nty, ntz;
+};
+
+KeyGeom key_geom(const Plan &p, double boxsize);
+
+// Floor cell of one coordinate, wrapped into [0, d), and the fractional part
+// (fieldize.cpp:66-75).  The floor is converted to a 32-bit integer as the
+// reference does; positions more than one box outside [0, box) take the
+// general modulo.
+__device__ __forceinline__ int cell_of(float p, double units, int d, double &frac) {
+  const double u = (double)p * units;
+  const double f = floor(u);
+  frac = u - f;
+  int c = (int)f;
+  if ((unsigned)c >= (unsigned)d) {
+    c %= d;
+    if (c < 0) c += d;
+  }
+  return c;
+}
+
+__device__ __forceinline__ uint32_t tile_key(const float *r, const KeyGeom &g) {
+  double fr;
+  const int cx = cell_of(r[0], g.units, g.d, fr);
+  const int cy = cell_of(r[1], g.units, g.d, fr);
+  const int cz = cell_of(r[2], g.units, g.d, fr);
+  return (uint32_t)((((cx >> g.sx) * g.nty + (cy >> g.sy)) * g.ntz) + (cz >> g.sz));
+}
+
+// Work item -> tile k and its particle range [pb, pe): tile k has m items
+// (item_start[k+1] - item_start[k]) that split its records evenly; j is the
+// item's number within the tile.  Returns false past the last item.
+struct ItemRange {
+  int64_t k;
+  uint32_t m, j, pb, pe;
+};
+__device__ __forceinline__ bool item_range(const uint32_t *__restrict__ item_start,
+                                           const uint32_t *__restrict__ tile_start, int64_t ks, uint32_t item,
+                                           ItemRange &r) {
+  if (item >= item_start[ks]) return false;
+  int64_t lo = 0, hi = ks;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (item_start[mid] <= item) lo = mid; else hi = mid;
+  }
+  r.k = lo;
+  r.m = item_start[lo + 1] - item_start[lo];
+  r.j = item - item_start[lo];
+  const uint32_t t0 = tile_start[lo], t1 = tile_start[lo + 1];
+  const uint32_t per = (t1 - t0 + r.m - 1) / r.m;
+  r.pb = min(t1, t0 + r.j * per);
+  r.pe = min(t1, r.pb + per);
+  return true;
+}
+
+// Sort the n particles of one call (n < 2^31) by tile into plan scratch and
+// build the work items (at most DEPOSIT_ITEM_MAX records each).  Records are
+// x, y, z, then the mass when masses != nullptr, then the particle's index in
+// [0, n) (as the bits of a float) when with_index.  every_tile: every tile
+// gets at least one item (the deposit then writes untouched tiles as
+// zeros); otherwise empty tiles get none.
+TileOrder sort_by_tile(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, const float *masses, bool with_index,
+                       bool every_tile);
+
+}  // namespace genpk

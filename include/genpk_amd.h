@@ -61,6 +61,15 @@ int genpk_clear(genpk_plan *plan);
  * (then every particle has mass `mass`). */
 int genpk_deposit(genpk_plan *plan, double boxsize, int64_t n, const float *pos, const float *masses, double mass);
 
+/* genpk_deposit that also keeps the particles' tile order (with each
+ * particle's index) in the plan, so that a following genpk_adjoint_gather
+ * of the same arrays (same pointers, n and boxsize, contents unchanged)
+ * skips its own partition.  Any later deposit drops the kept order.  Kept
+ * only when n <= 2^31.  No counterpart in the reference (which has no
+ * gradients). */
+int genpk_deposit_retain(genpk_plan *plan, double boxsize, int64_t n, const float *pos, const float *masses,
+                         double mass);
+
 /* Write the real-space density into out, a padded grid [d][d][out_rowlen]
  * of the plan's precision; accumulate != 0 adds to out.  Pad columns
  * (z >= d) are left untouched. */
@@ -69,6 +78,12 @@ int genpk_export_density(genpk_plan *plan, void *out, int64_t out_rowlen, int ac
 /* Replace the density by a caller grid [d][d][2(d/2+1)] of the plan's
  * precision (pads ignored); later deposits add to it. */
 int genpk_import_density(genpk_plan *plan, const void *grid);
+
+/* Replace the plan's content by a caller half spectrum [d][d][d/2+1]
+ * (complex of the plan's precision), e.g. one saved by an earlier
+ * genpk_r2c; the plan then holds a spectrum (for genpk_bin or the
+ * adjoint). */
+int genpk_import_spectrum(genpk_plan *plan, const void *spectrum);
 
 /* In-place forward r2c of the density: the FFTW step of gen-pk.cpp:233.
  * Afterwards the plan's grid holds the complex half spectrum
@@ -88,20 +103,28 @@ int genpk_bin(genpk_plan *a, genpk_plan *b, int nrbins, double *power_aa, double
               int *count, double *keffs, double ma, double mb);
 
 /* Fused forward: r2c of a (and b) whose last FFT pass bins the modes while
- * they are in LDS; the spectrum is not kept (the plan's grid is left in an
- * intermediate state).  Outputs as genpk_bin. */
+ * they are in LDS; the spectrum is not kept.  The plan's grid is left in
+ * the state after the z and y passes: the plan then holds no density (a
+ * deposit starts from zero), but genpk_adjoint_density can start from it.
+ * Outputs as genpk_bin. */
 int genpk_power(genpk_plan *a, genpk_plan *b, int nrbins, double *power_aa, double *power_bb, double *power_ab,
                 int *count, double *keffs, double ma, double mb);
 
-/* Adjoint of the auto spectrum.  The plan must hold the forward spectrum
- * (genpk_r2c).  lambda: device double [nrbins] = dL/dP_b; m: the total
- * mass the forward normalised by.  Afterwards the plan's grid holds
- * dL/drho on the padded real layout. */
+/* Adjoint of the auto spectrum (no counterpart in the reference).  The
+ * plan must hold one of: the forward spectrum (genpk_r2c or
+ * genpk_import_spectrum), the state left by genpk_power with nothing
+ * deposited since, or the forward density (deposits and/or an imported
+ * density, as before genpk_power; the forward transform is then redone).
+ * lambda: device double [nrbins] = dL/dP_b; m: the total mass the forward
+ * normalised by.  Afterwards the plan's grid holds dL/drho on the padded
+ * real layout. */
 int genpk_adjoint_density(genpk_plan *plan, int nrbins, const double *lambda, double m);
 
 /* Chain dL/drho to the particles: grad_pos float [n][3] = dL/dpos,
  * grad_mass float [n] = dL/dm_j (may be NULL).  pos, masses, mass, boxsize
- * as given to genpk_deposit. */
+ * as given to genpk_deposit.  Uses the tile order kept by
+ * genpk_deposit_retain for the same arrays, else partitions the particles
+ * itself (and keeps that order). */
 int genpk_adjoint_gather(genpk_plan *plan, double boxsize, int64_t n, const float *pos, const float *masses,
                          double mass, float *grad_pos, float *grad_mass);
 
