@@ -38,27 +38,94 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 comparison pass")
     ap.add_argument("--cpu-sample-log2", type=int, default=24)
+    ap.add_argument("--no-backward", action="store_true", help="skip the forward+backward measurement")
+    ap.add_argument("--backward-steps", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 4 GiB device copy (reference kernel for HBM byte counters)")
     return ap.parse_args()
 
 
-def stage_bytes(name, n, c, p_pad, cells_total, rec):
+def stage_bytes(name, n, c, p_pad, geo, rec):
     """Algorithmic HBM bytes of one launch of a stage: what the stage must
-    read and write once for its function (the design document lists them)."""
+    read and write once for its function.  geo: deposit tiling of the plan
+    (tiles K, halo'd cells per tile, primary cells per tile)."""
+    K, cells, prim = geo
     if name.startswith("sort_upsweep"):
-        return rec * n
+        return rec * n                                   # read the records
     if name.startswith("sort_downsweep"):
-        return 2 * rec * n
+        return 2 * rec * n                               # read + write the records
     if name == "deposit_tiles":
-        return rec * n + c * cells_total
-    if name == "fold":
-        return c * cells_total + c * p_pad
-    if name in ("fft_z", "fft_y", "fft_x", "ifft_z", "ifft_y", "ifft_x"):
+        return rec * n + c * K * cells                   # read records, write the halo'd tiles
+    if name == "halo_fold":
+        return 3 * c * K * (cells - prim)                # read + zero the halo cells, update the faces
+    if name == "fold_fft_z":
+        return c * K * prim + c * p_pad                  # read the primary tile cells, write the padded grid
+    if name in ("fft_z", "fft_y", "fft_x", "ifft_z", "ifft_y", "ifft_x", "adjoint_x"):
         return 2 * c * p_pad
     if name == "xpass_bin":
         return c * p_pad
+    if name == "adjoint_gather":
+        return (rec + 4) * n + c * K * cells + 12 * n    # records with index, G tiles, gradient write
     return 0
+
+
+def tile_geometry(d):
+    """Deposit tiling of a dims^3 plan: edges min(16|32|32, largest power of two <= d)."""
+    def p2(cap):
+        t = 1
+        while t * 2 <= min(cap, d):
+            t *= 2
+        return t
+    tx, ty, tz = p2(16), p2(32), p2(32)
+    K = (-(-d // tx)) * (-(-d // ty)) * (-(-d // tz))
+    return K, (tx + 1) * (ty + 1) * (tz + 1), tx * ty * tz
+
+
+def fwd_bwd(genpk_amd, torch, pos, box, d, nbins, steps, prec):
+    """Forward + backward of the band-power chi^2 loss (target: the forward
+    P times 1.05) with the recompute policy and a masses tensor (all ones)
+    that receives gradients, against the forward on the same inputs."""
+    from genpk_amd.autograd import chi2_loss, power_spectrum
+    n = pos.shape[0]
+    plan = genpk_amd.Plan(d, prec)
+    masses = torch.ones(n, dtype=torch.float32, device=pos.device)
+    plan.clear()
+    plan.deposit(pos, box, masses)
+    P0, cnt, _ = plan.power(nbins, mass=float(n))
+    target = P0 * 1.05
+
+    def fwd():
+        plan.clear()
+        plan.deposit(pos, box, masses)
+        return plan.power(nbins, mass=float(n))[0]
+
+    def fb(policy):
+        tp = pos.detach().requires_grad_(True)
+        tm = masses.detach().requires_grad_(True)
+        P = power_spectrum(tp, box, d, nbins, masses=tm, total_mass=float(n), plan=plan, recompute=policy)
+        return torch.autograd.grad(chi2_loss(P, target, cnt), (tp, tm))
+
+    def timed(f, *a):
+        f(*a)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            f(*a)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    t_f = timed(fwd)
+    out = {"loss": "sum_b (P_b - T_b)^2 / sigma_b^2, sigma_b = T_b sqrt(2/count_b), T = 1.05 P",
+           "grads": "positions and masses", "steps": steps, "ms_fwd_same_inputs": round(t_f, 3)}
+    for policy in ("always", "auto"):
+        t = timed(fb, policy)
+        out[f"ms_fwd_bwd_{policy}"] = round(t, 3)
+        out[f"ratio_{policy}"] = round(t / t_f, 3)
+    out["policy_of_ratio"] = "always (deposit + transform recomputed in the backward)"
+    out["ratio"] = out["ratio_always"]
+    del plan
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(pos_dev, box, dims, nbins, n_total, sample_log2):
@@ -138,12 +205,12 @@ def main():
 
     c = 4 if args.prec == "f32" else 8
     p_pad = d * d * 2 * (d // 2 + 1)
-    tg_cells = (-(-d // 16)) * (-(-d // 32)) * (-(-d // 32)) * 17 * 33 * 33
+    geo = tile_geometry(d)
     rec = 12
     per_stage = {}
     for name, (tot, launches) in stages.items():
         avg = tot / max(launches, 1)
-        b = stage_bytes(name, n, c, p_pad, tg_cells, rec)
+        b = stage_bytes(name, n, c, p_pad, geo, rec)
         per_stage[name] = {"ms": round(avg, 4), "launches_per_step": launches / args.steps,
                            "alg_bytes": b, "GBps": round(b / (avg * 1e-3) / 1e9, 1) if avg > 0 and b else None}
     dom = max(per_stage, key=lambda k: per_stage[k]["ms"] * per_stage[k]["launches_per_step"])
@@ -182,6 +249,10 @@ def main():
         torch.cuda.synchronize()
         del a, b
 
+    adj = None
+    if not args.no_backward:
+        adj = fwd_bwd(genpk_amd, torch, pos, args.box, d, nbins, args.backward_steps, args.prec)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pos, args.box, d, nbins, n, args.cpu_sample_log2)
@@ -209,6 +280,7 @@ def main():
                      "frac_of_hbm_peak": round(pipeline_gbs / HBM_PEAK_GBS, 4)},
         "stages": per_stage,
         "accuracy": check,
+        "fwd_bwd": adj,
         "cpu_baseline": cpu,
         "input_gen_s": round(t_gen, 2),
     }

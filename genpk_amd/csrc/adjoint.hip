@@ -84,8 +84,9 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_adjoint_x(typename Cx<T>::C *__
   C *buf = reinterpret_cast<C *>(lds_raw);  // [n][B]
   const int n = rp.n;
   const int B = 1 << lgb;
-  const int64_t y = blockIdx.x / nkb;
-  const int kb = blockIdx.x - (int)(y * nkb);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t y = blk / nkb;
+  const int kb = (int)(blk - y * nkb);
   const int64_t kz0 = (int64_t)kb * B;
   const int nb = (int)min((int64_t)B, hz - kz0);
   const int64_t estride = d * hz;

@@ -196,7 +196,9 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
   const int my_b = threadIdx.x % B;
   const int per = (n + tpc - 1) / tpc;
   const int e0 = (threadIdx.x / B) * per;
-  for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+  for (int64_t v = blockIdx.x; v < nblk; v += gridDim.x) {
+    // a grid that is a multiple of the XCD count keeps v on this workgroup's XCD
+    const int64_t blk = xcd_block((uint32_t)v, (uint32_t)nblk);
     const int64_t y = blk / nkb;
     const int kb = (int)(blk % nkb);
     const int64_t kz0 = (int64_t)kb * B;

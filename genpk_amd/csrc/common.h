@@ -48,4 +48,16 @@ template <> struct cplx_of<double> { using type = double2; };
 
 __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Workgroups are dispatched to the 8 XCDs (each with its own L2) round
+// robin by workgroup id.  xcd_block maps the dispatched id p of a grid of g
+// workgroups to a logical block so that every XCD runs one contiguous range
+// of logical blocks, in order: neighbouring blocks, which share cache lines
+// at their edges, then meet in the same L2.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t p, uint32_t g) {
+  const uint32_t q = g / NUM_XCD, r = g % NUM_XCD;
+  const uint32_t x = p % NUM_XCD, slot = p / NUM_XCD;
+  return x * q + min(x, r) + slot;
+}
+
 }  // namespace genpk

@@ -248,21 +248,26 @@ void deposit_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass) {
   P.mass = mass;
   P.store = store;
   const size_t lds = (size_t)tg.cells * 8;
-  Stage st(p, "deposit_tiles");
-  if (p.prec == 4) {
-    hipLaunchKernelGGL((k_deposit<float, NM, RW>), dim3((unsigned)max_items), dim3(DEP_THREADS), lds, s, sorted,
-                       tile_start, item_start, part_start, P, p.tiles.as<float>(), partials);
+  {
+    Stage st(p, "deposit_tiles");
+    if (p.prec == 4)
+      hipLaunchKernelGGL((k_deposit<float, NM, RW>), dim3((unsigned)max_items), dim3(DEP_THREADS), lds, s, sorted,
+                         tile_start, item_start, part_start, P, p.tiles.as<float>(), partials);
+    else
+      hipLaunchKernelGGL((k_deposit<double, NM, RW>), dim3((unsigned)max_items), dim3(DEP_THREADS), lds, s, sorted,
+                         tile_start, item_start, part_start, P, p.tiles.as<double>(), partials);
     GENPK_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_partials<float>, dim3(1024), dim3(256), 0, s, nheavy, heavy, item_start, part_start,
-                       partials, P, p.tiles.as<float>());
-  } else {
-    hipLaunchKernelGGL((k_deposit<double, NM, RW>), dim3((unsigned)max_items), dim3(DEP_THREADS), lds, s, sorted,
-                       tile_start, item_start, part_start, P, p.tiles.as<double>(), partials);
-    GENPK_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_partials<double>, dim3(1024), dim3(256), 0, s, nheavy, heavy, item_start, part_start,
-                       partials, P, p.tiles.as<double>());
   }
-  GENPK_LAUNCH_CHECK();
+  {
+    Stage st(p, "reduce_partials");
+    if (p.prec == 4)
+      hipLaunchKernelGGL(k_reduce_partials<float>, dim3(1024), dim3(256), 0, s, nheavy, heavy, item_start, part_start,
+                         partials, P, p.tiles.as<float>());
+    else
+      hipLaunchKernelGGL(k_reduce_partials<double>, dim3(1024), dim3(256), 0, s, nheavy, heavy, item_start,
+                         part_start, partials, P, p.tiles.as<double>());
+    GENPK_LAUNCH_CHECK();
+  }
   {
     Stage hst(p, "halo_fold");
     const FoldGeom fg = fold_geom(p);

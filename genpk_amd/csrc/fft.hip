@@ -38,7 +38,7 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
   extern __shared__ unsigned char lds_raw[];
   C *buf = reinterpret_cast<C *>(lds_raw);
   const int M = rp.n;
-  const int64_t row0 = (int64_t)blockIdx.x * L;
+  const int64_t row0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * L;
   const int nl = (int)min((int64_t)L, nrows - row0);
   const int tot = L * M;
   C v[MAXE];
@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_c2r_z(T *__restrict__ field, in
   C *buf = reinterpret_cast<C *>(lds_raw);  // [L][M+1]
   const int M = rp.n;
   const int W = M + 1;
-  const int64_t row0 = (int64_t)blockIdx.x * L;
+  const int64_t row0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * L;
   const int nl = (int)min((int64_t)L, nrows - row0);
   for (int i = threadIdx.x; i < L * W; i += FFT_THREADS) {
     const int l = (int)fdM1.div((uint32_t)i), k = i - l * W;
@@ -165,8 +165,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_cols(typename Cx<T>::C *__restr
   C *buf = reinterpret_cast<C *>(lds_raw);  // [n][B]
   const int n = rp.n;
   const int B = 1 << lgb;
-  const int64_t outer = blockIdx.x / nkb;
-  const int kb = blockIdx.x - (int)(outer * nkb);
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t outer = blk / nkb;
+  const int kb = (int)(blk - outer * nkb);
   const int64_t kz0 = (int64_t)kb * B;
   const int nb = (int)min((int64_t)B, hz - kz0);
   const int64_t estride = axis == 1 ? hz : d * hz;

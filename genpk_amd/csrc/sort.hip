@@ -91,7 +91,8 @@ __global__ void __launch_bounds__(SORT_THREADS) k_upsweep(Src<NM, NI> src, Chunk
   constexpr int UNR = 8;
   extern __shared__ uint32_t hist[];
   int64_t b, e;
-  if (!cm.range(blockIdx.x, b, e)) return;
+  const uint32_t g = blockIdx.x;
+  if (!cm.range(g, b, e)) return;
   for (int i = threadIdx.x; i < nb; i += SORT_THREADS) hist[i] = 0;
   __syncthreads();
   for (int64_t base = b; base < e; base += SORT_THREADS * UNR) {
@@ -117,7 +118,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_upsweep(Src<NM, NI> src, Chunk
     }  }
   __syncthreads();
   for (int i = threadIdx.x; i < nb; i += SORT_THREADS)
-    counts[(int64_t)i * sd + (int64_t)blockIdx.x * sg] = hist[i];
+    counts[(int64_t)i * sd + (int64_t)g * sg] = hist[i];
 }
 
 // Stable partition of each chunk by digit into dst at offsets[dg*sd + g*sg].
@@ -142,10 +143,11 @@ __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<NM, NI> src, flo
   __shared__ uint32_t wtot[SORT_WAVES];
 
   int64_t cb, ce;
-  if (!cm.range(blockIdx.x, cb, ce)) return;
+  const uint32_t g = blockIdx.x;
+  if (!cm.range(g, cb, ce)) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int i = threadIdx.x; i < nb; i += SORT_THREADS)
-    gbase[i] = offsets[(int64_t)i * sd + (int64_t)blockIdx.x * sg];
+    gbase[i] = offsets[(int64_t)i * sd + (int64_t)g * sg];
   for (int i = threadIdx.x; i < SORT_WAVES * nb; i += SORT_THREADS) whist[i] = 0;
 
   float rec[SORT_PER_LANE][RW];
