@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--no-backward", action="store_true", help="skip the forward+backward measurement")
     ap.add_argument("--backward-steps", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run a plain 4 GiB device copy (reference kernel for HBM byte counters)")
+                    help="also run plain copy kernels of known size (references for the HBM byte counters)")
     return ap.parse_args()
 
 
@@ -242,12 +242,20 @@ def main():
         torch.cuda.empty_cache()
 
     if args.copy_calibration:
+        # reference kernels for the HBM byte counters, with the access width of
+        # this library's kernels: k_fold (export_density) copies the d^3 cells
+        # of the tile store into a padded grid with 4- or 8-byte accesses,
+        # reading and writing c*d^3 bytes; a torch elementwise add over 2^30
+        # floats (16-byte accesses) reads and writes 4 GiB
+        plan.clear()
+        plan.deposit(pos, args.box, mass=1.0)
+        out = plan.export_density()
         a = torch.empty(1 << 30, dtype=torch.float32, device=dev).fill_(1.0)
         b = torch.empty_like(a)
         torch.cuda.synchronize()
-        b.copy_(a)
+        torch.add(a, 1.0, out=b)
         torch.cuda.synchronize()
-        del a, b
+        del a, b, out
 
     adj = None
     if not args.no_backward:
