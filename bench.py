@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 comparison pass")
     ap.add_argument("--cpu-sample-log2", type=int, default=24)
     ap.add_argument("--no-backward", action="store_true", help="skip the forward+backward measurement")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched (64 uneven realisations) measurement")
+    ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--backward-steps", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run plain copy kernels of known size (references for the HBM byte counters)")
@@ -126,6 +128,60 @@ def fwd_bwd(genpk_amd, torch, pos, box, d, nbins, steps, prec):
     del plan
     torch.cuda.empty_cache()
     return out
+
+
+def batched(genpk_amd, torch, prec, count, reps=5):
+    """Uneven batch: `count` realisations, realisation r with seed 100 + r,
+    n_side log-uniform in [64, 256] (n_side^3 particles), dims uniform over
+    {128, 256, 384, 512}, 0.3-cell Gaussian lattice in a 100 Mpc/h box,
+    nrbins = dims.  Plain: one plan per dims, each realisation run alone
+    (clear + deposit + fused power, then synchronize), times summed.
+    Batched: the autotuned hipGraph over all realisations, one launch."""
+    from genpk_amd import batch as gb
+    from genpk_amd import synth
+    box = 100.0
+    jobs = []
+    for ns, d, sd in synth.uneven_realisations(count):
+        pos = synth.gaussian_lattice_torch(ns, box, d, 0.3, seed=sd)
+        jobs.append(dict(pos=pos, boxsize=box, dims=d, nrbins=d, total_mass=float(pos.shape[0]), precision=prec))
+    n_tot = sum(j["pos"].shape[0] for j in jobs)
+    plans = {d: genpk_amd.Plan(d, prec) for d in sorted({j["dims"] for j in jobs})}
+
+    def plain():
+        outs = []
+        for j in jobs:
+            p = plans[j["dims"]]
+            p.clear()
+            p.deposit(j["pos"], box)
+            outs.append(p.power(j["nrbins"], mass=j["total_mass"])[0])
+            torch.cuda.synchronize()
+        return outs
+
+    plain()
+    t_plain = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ref = plain()
+        t_plain.append(time.perf_counter() - t0)
+    t_plain = sorted(t_plain)[1]
+    del plans
+    torch.cuda.empty_cache()
+    best, rep = gb.tune(jobs, use_cache=False)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        best.launch()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    t_b = sorted(ts)[reps // 2]
+    dev = max(float(((best.results(i)[0] - ref[i]).abs() / ref[i].abs().clamp(min=1e-300)).max())
+              for i in range(len(jobs)))
+    best.close()
+    torch.cuda.empty_cache()
+    return {"realisations": count, "particles": n_tot, "dims": sorted({j["dims"] for j in jobs}),
+            "ms_plain_sum": round(t_plain * 1e3, 3), "ms_batched": round(t_b * 1e3, 3),
+            "speedup": round(t_plain / t_b, 3), "Gparticles_per_s_batched": round(n_tot / t_b / 1e9, 3),
+            "autotune": rep, "max_rel_dev_from_plain": dev}
 
 
 def cpu_baseline(pos_dev, box, dims, nbins, n_total, sample_log2):
@@ -261,6 +317,10 @@ def main():
     if not args.no_backward:
         adj = fwd_bwd(genpk_amd, torch, pos, args.box, d, nbins, args.backward_steps, args.prec)
 
+    bat = None
+    if not args.no_batched:
+        bat = batched(genpk_amd, torch, args.prec, args.batch_count)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pos, args.box, d, nbins, n, args.cpu_sample_log2)
@@ -289,6 +349,7 @@ def main():
         "stages": per_stage,
         "accuracy": check,
         "fwd_bwd": adj,
+        "batched": bat,
         "cpu_baseline": cpu,
         "input_gen_s": round(t_gen, 2),
     }

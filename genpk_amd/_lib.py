@@ -37,6 +37,13 @@ SIGNATURES = {
     "genpk_power": (_c_int, [_c_vp, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_dbl]),
     "genpk_adjoint_density": (_c_int, [_c_vp, _c_int, _c_vp, _c_dbl]),
     "genpk_adjoint_gather": (_c_int, [_c_vp, _c_dbl, _c_i64, _c_vp, _c_vp, _c_dbl, _c_vp, _c_vp]),
+    "genpk_batch_create": (_c_int, [ctypes.POINTER(_c_vp), _c_int, _c_vp]),
+    "genpk_batch_add": (_c_int, [_c_vp, _c_i64, _c_int, _c_dbl, _c_i64, _c_vp, _c_vp, _c_dbl, _c_int, _c_dbl, _c_vp,
+                                 _c_vp, _c_vp]),
+    "genpk_batch_build": (_c_int, [_c_vp, _c_vp]),
+    "genpk_batch_launch": (_c_int, [_c_vp]),
+    "genpk_batch_branch_of": (_c_int, [_c_vp, _c_i64, _ip]),
+    "genpk_batch_destroy": (_c_int, [_c_vp]),
     "genpk_timing_enable": (_c_int, [_c_vp, _c_int]),
     "genpk_timing_read": (_c_int, [_c_vp, ctypes.c_char_p, _c_sz, _dp, _ip, _c_int, _ip]),
     "genpk_nexttwo": (_c_int, [_c_int]),

@@ -44,68 +44,23 @@ __device__ __forceinline__ double deconv_of(const float *__restrict__ wtab, int6
   return w * w;                      // its square, as applied per mode
 }
 
-// Data-independent per-bin mode count and sum of h*|k| over the half spectrum.
-__global__ void __launch_bounds__(BIN_THREADS) k_counts(int64_t d, int nrbins, const int *__restrict__ q2bin,
-                                                         uint32_t *__restrict__ part_cnt,
-                                                         double *__restrict__ part_keff) {
-  extern __shared__ unsigned char lds_raw[];
-  double *kb = reinterpret_cast<double *>(lds_raw);
-  uint32_t *cb = reinterpret_cast<uint32_t *>(kb + nrbins);
-  for (int i = threadIdx.x; i < nrbins; i += BIN_THREADS) {
-    kb[i] = 0;
-    cb[i] = 0;
-  }
-  __syncthreads();
+// Mode count per |k|^2 = q over the half spectrum, weighted by the Hermitian
+// weight h (1 on the planes kz = 0 and kz = d/2, else 2): with r2[a] the
+// number of (x, y) array indices whose kx^2 + ky^2 = a,
+//   qcnt[q] = sum_kz h(kz) r2[q - kz^2].
+// Integer sums: the result is independent of scheduling.
+__global__ void k_qcount(int64_t d, int64_t qmax, const uint32_t *__restrict__ r2, int64_t amax,
+                         uint32_t *__restrict__ qcnt) {
   const int64_t hz = d / 2 + 1;
-  const int64_t ncol = d * d;
-  for (int64_t c = (int64_t)blockIdx.x * BIN_THREADS + threadIdx.x; c < ncol; c += (int64_t)gridDim.x * BIN_THREADS) {
-    const int64_t kx = c / d, ky = c % d;
-    const int64_t a = kval(kx, d) * kval(kx, d) + kval(ky, d) * kval(ky, d);
-    int cur = -1;
-    uint32_t cc = 0;
-    double ck = 0;
-    for (int64_t kz = 0; kz < hz; kz++) {
-      const int64_t q = a + kz * kz;
-      if (q == 0) continue;
-      const int b = q2bin[q];
-      const uint32_t h = (kz == 0 || kz == d / 2) ? 1u : 2u;
-      if (b != cur) {
-        if (cur >= 0) {
-          atomicAdd(&cb[cur], cc);
-          atomicAdd(&kb[cur], ck);
-        }
-        cur = b;
-        cc = 0;
-        ck = 0;
-      }
-      cc += h;
-      ck += (double)h * sqrt((double)q);
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q <= qmax; q += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t s = 0;
+    for (int64_t kz = 0; kz < hz && kz * kz <= q; kz++) {
+      const int64_t a = q - kz * kz;
+      if (a > amax) continue;
+      s += ((kz == 0 || kz == d / 2) ? 1u : 2u) * r2[a];
     }
-    if (cur >= 0) {
-      atomicAdd(&cb[cur], cc);
-      atomicAdd(&kb[cur], ck);
-    }
+    qcnt[q] = s;
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < nrbins; i += BIN_THREADS) {
-    part_cnt[(int64_t)blockIdx.x * nrbins + i] = cb[i];
-    part_keff[(int64_t)blockIdx.x * nrbins + i] = kb[i];
-  }
-}
-
-__global__ void k_counts_reduce(int nrbins, int G, const uint32_t *__restrict__ part_cnt,
-                                const double *__restrict__ part_keff, int *__restrict__ count,
-                                double *__restrict__ keffsum) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nrbins) return;
-  uint64_t c = 0;
-  double k = 0;
-  for (int g = 0; g < G; g++) {
-    c += part_cnt[(int64_t)g * nrbins + b];
-    k += part_keff[(int64_t)g * nrbins + b];
-  }
-  count[b] = (int)c;
-  keffsum[b] = k;
 }
 
 // Accumulates one spectrum contribution run-by-run in registers.
@@ -396,21 +351,46 @@ BinTables &bin_tables(Plan &p, int nrbins) {
     p.wtab.ensure(d * 4);
     GENPK_HIP(hipMemcpy(p.wtab.ptr, w.data(), d * 4, hipMemcpyHostToDevice));
   }
-  // counts and keffs, once
-  const int G = std::min<int64_t>(2 * num_cus(), std::max<int64_t>(1, ceil_div(d * d, BIN_THREADS)));
-  DevBuf pc, pk;
-  pc.ensure((size_t)G * nrbins * 4);
-  pk.ensure((size_t)G * nrbins * 8);
-  bt->count.ensure((size_t)nrbins * 4);
-  bt->keffsum.ensure((size_t)nrbins * 8);
-  const size_t lds = (size_t)nrbins * 12;
-  hipLaunchKernelGGL(k_counts, dim3(G), dim3(BIN_THREADS), lds, p.stream, d, nrbins, bt->q2bin.as<int>(),
-                     pc.as<uint32_t>(), pk.as<double>());
-  GENPK_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_counts_reduce, dim3((unsigned)ceil_div(nrbins, 256)), dim3(256), 0, p.stream, nrbins, G,
-                     pc.as<uint32_t>(), pk.as<double>(), bt->count.as<int>(), bt->keffsum.as<double>());
-  GENPK_LAUNCH_CHECK();
-  GENPK_HIP(hipStreamSynchronize(p.stream));  // pc/pk are released on return
+  // counts and keffs, once: per-q mode counts on the device, then per-bin
+  // sums on the host in ascending q (deterministic)
+  {
+    const int64_t amax = 2 * h * h;
+    std::vector<uint32_t> r2((size_t)amax + 1, 0);
+    for (int64_t x = 0; x < d; x++) {
+      const int64_t kx = x <= h ? x : x - d;
+      for (int64_t y = 0; y < d; y++) {
+        const int64_t ky = y <= h ? y : y - d;
+        r2[(size_t)(kx * kx + ky * ky)]++;
+      }
+    }
+    DevBuf dr2, dq;
+    dr2.ensure(r2.size() * 4);
+    dq.ensure(((size_t)qmax + 1) * 4);
+    GENPK_HIP(hipMemcpy(dr2.ptr, r2.data(), r2.size() * 4, hipMemcpyHostToDevice));
+    const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(qmax + 1, 256), 65536);
+    hipLaunchKernelGGL(k_qcount, dim3(grid), dim3(256), 0, p.stream, d, qmax, dr2.as<uint32_t>(), amax,
+                       dq.as<uint32_t>());
+    GENPK_LAUNCH_CHECK();
+    std::vector<uint32_t> qcnt((size_t)qmax + 1);
+    GENPK_HIP(hipMemcpyAsync(qcnt.data(), dq.ptr, qcnt.size() * 4, hipMemcpyDeviceToHost, p.stream));
+    GENPK_HIP(hipStreamSynchronize(p.stream));
+    std::vector<int64_t> cnt(nrbins, 0);
+    std::vector<double> ke(nrbins, 0.0);
+    for (int64_t q = 1; q <= qmax; q++) {
+      if (!qcnt[q]) continue;
+      cnt[tab[q]] += qcnt[q];
+      ke[tab[q]] += (double)qcnt[q] * sqrt((double)q);
+    }
+    std::vector<int> cnt32(nrbins);
+    for (int b = 0; b < nrbins; b++) {
+      GENPK_REQUIRE(cnt[b] <= 2147483647, "a bin holds more than 2^31-1 modes");
+      cnt32[b] = (int)cnt[b];
+    }
+    bt->count.ensure((size_t)nrbins * 4);
+    bt->keffsum.ensure((size_t)nrbins * 8);
+    GENPK_HIP(hipMemcpy(bt->count.ptr, cnt32.data(), (size_t)nrbins * 4, hipMemcpyHostToDevice));
+    GENPK_HIP(hipMemcpy(bt->keffsum.ptr, ke.data(), (size_t)nrbins * 8, hipMemcpyHostToDevice));
+  }
   bt->counts_ready = true;
   BinTables &ref = *bt;
   p.bins[nrbins] = std::move(bt);

@@ -79,30 +79,10 @@ static bool density_state(const Plan &p) {
   return p.fstate == FieldState::None || p.fstate == FieldState::Real || p.fstate == FieldState::YDone;
 }
 
-static bool fft_ok(const Plan &p) {
+bool plan_fft_ok(const Plan &p) {
   return (p.d % 2 == 0) && p.d >= 2 && fft_length_ok(p.d) && fft_length_ok(p.d / 2);
 }
 
-template <typename F>
-static int guarded(F &&f) {
-  try {
-    f();
-    g_last_error.clear();
-    return 0;
-  } catch (const Error &e) {
-    g_last_error = e.what();
-    return 1;
-  } catch (const std::bad_alloc &) {
-    g_last_error = "host allocation failed";
-    return 2;
-  } catch (const std::exception &e) {
-    g_last_error = e.what();
-    return 3;
-  } catch (...) {
-    g_last_error = "unknown error";
-    return 4;
-  }
-}
 
 // ---- cached host-surface plans for the compatibility shims ----
 struct CompatCache {
@@ -257,7 +237,7 @@ int genpk_r2c(genpk_plan *plan) {
   return guarded([&] {
     GENPK_REQUIRE(plan && plan->p, "null plan");
     Plan &p = *plan->p;
-    GENPK_REQUIRE(fft_ok(p), "the FFT needs an even dims with dims/2 = 2^a 3^b 5^c");
+    GENPK_REQUIRE(plan_fft_ok(p), "the FFT needs an even dims with dims/2 = 2^a 3^b 5^c");
     GENPK_REQUIRE(density_state(p), "the plan holds no density");
     r2c(p);
   });
@@ -266,7 +246,7 @@ int genpk_r2c(genpk_plan *plan) {
 int genpk_c2r(genpk_plan *plan) {
   return guarded([&] {
     GENPK_REQUIRE(plan && plan->p, "null plan");
-    GENPK_REQUIRE(fft_ok(*plan->p), "the FFT needs an even dims with dims/2 = 2^a 3^b 5^c");
+    GENPK_REQUIRE(plan_fft_ok(*plan->p), "the FFT needs an even dims with dims/2 = 2^a 3^b 5^c");
     c2r(*plan->p);
   });
 }
@@ -286,7 +266,7 @@ int genpk_power(genpk_plan *a, genpk_plan *b, int nrbins, double *power_aa, doub
     GENPK_REQUIRE(a && a->p, "null plan");
     Plan &pa = *a->p;
     Plan *pb = b ? b->p.get() : nullptr;
-    GENPK_REQUIRE(fft_ok(pa), "the FFT needs an even dims with dims/2 = 2^a 3^b 5^c");
+    GENPK_REQUIRE(plan_fft_ok(pa), "the FFT needs an even dims with dims/2 = 2^a 3^b 5^c");
     GENPK_REQUIRE(density_state(pa), "plan a holds no density");
     if (pb) GENPK_REQUIRE(density_state(*pb), "plan b holds no density");
     (void)bin_tables(pa, nrbins);
@@ -303,7 +283,7 @@ int genpk_power(genpk_plan *a, genpk_plan *b, int nrbins, double *power_aa, doub
 int genpk_adjoint_density(genpk_plan *plan, int nrbins, const double *lambda, double m) {
   return guarded([&] {
     GENPK_REQUIRE(plan && plan->p && lambda, "null argument");
-    GENPK_REQUIRE(fft_ok(*plan->p), "the FFT needs an even dims with dims/2 = 2^a 3^b 5^c");
+    GENPK_REQUIRE(plan_fft_ok(*plan->p), "the FFT needs an even dims with dims/2 = 2^a 3^b 5^c");
     adjoint_density(*plan->p, nrbins, lambda, m);
   });
 }
@@ -446,7 +426,7 @@ int genpk_fft_r2c_3d_inplace(int64_t dims, double *field) {
   return guarded([&] {
     GENPK_REQUIRE(field, "null field");
     Plan &pa = compat_plan(cc.a, dims);
-    GENPK_REQUIRE(fft_ok(pa), "the FFT needs an even dims with dims/2 = 2^a 3^b 5^c");
+    GENPK_REQUIRE(plan_fft_ok(pa), "the FFT needs an even dims with dims/2 = 2^a 3^b 5^c");
     const size_t bytes = pa.field_elems * 8;
     GENPK_HIP(hipMemcpy(pa.field.ptr, field, bytes, hipMemcpyHostToDevice));
     pa.fstate = FieldState::Real;

@@ -5,6 +5,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <string>
+#include <new>
 #include <stdexcept>
 
 namespace genpk {
@@ -31,6 +32,29 @@ void set_last_error(const std::string &msg);
   } while (0)
 
 #define GENPK_LAUNCH_CHECK() GENPK_HIP(hipGetLastError())
+
+// Body of every C entry point: runs f, maps exceptions to a nonzero status
+// and the thread's last-error message.
+template <typename F>
+int guarded(F &&f) {
+  try {
+    f();
+    set_last_error("");
+    return 0;
+  } catch (const Error &e) {
+    set_last_error(e.what());
+    return 1;
+  } catch (const std::bad_alloc &) {
+    set_last_error("host allocation failed");
+    return 2;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return 3;
+  } catch (...) {
+    set_last_error("unknown error");
+    return 4;
+  }
+}
 
 // Fixed-point scale of the LDS deposit accumulators: CIC weights in [0, 1]
 // are accumulated as unsigned 64-bit integers with 36 fractional bits

@@ -279,7 +279,23 @@ void deposit_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass) {
   }
 }
 
+__global__ void k_fill_zero(uint32_t *__restrict__ a, int64_t words) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = 0u;
+}
+
 }  // namespace
+
+// Zero a device buffer with a kernel (a memset node replays wrongly inside
+// captured graphs on this platform; a kernel is safe there).
+void fill_zero(void *ptr, size_t bytes, hipStream_t s) {
+  GENPK_REQUIRE(bytes % 4 == 0, "fill_zero: size must be a multiple of 4");
+  const int64_t words = (int64_t)(bytes / 4);
+  if (words == 0) return;
+  const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(words, 256), 65536);
+  hipLaunchKernelGGL(k_fill_zero, dim3(grid), dim3(256), 0, s, static_cast<uint32_t *>(ptr), words);
+  GENPK_LAUNCH_CHECK();
+}
 
 FoldGeom fold_geom(const Plan &p) {
   const TileGeom &tg = p.tg;
@@ -333,7 +349,7 @@ void fold_tiles(Plan &p, void *out, int64_t out_rowlen, bool accumulate) {
   GENPK_REQUIRE(out_rowlen >= p.d, "output row length shorter than dims");
   if (!p.tiles_live) {
     // nothing deposited: the density is zero
-    if (!accumulate) GENPK_HIP(hipMemsetAsync(out, 0, (size_t)p.d * p.d * out_rowlen * p.prec, p.stream));
+    if (!accumulate) fill_zero(out, (size_t)p.d * p.d * out_rowlen * p.prec, p.stream);
     return;
   }
   const FoldGeom F = fold_geom(p);
@@ -354,7 +370,7 @@ void fold_tiles(Plan &p, void *out, int64_t out_rowlen, bool accumulate) {
 void tiles_to_field(Plan &p) {
   if (!p.tiles_live) {
     if (p.fstate != FieldState::Real) {
-      GENPK_HIP(hipMemsetAsync(p.field.ptr, 0, p.field_elems * p.prec, p.stream));
+      fill_zero(p.field.ptr, p.field_elems * p.prec, p.stream);
       p.fstate = FieldState::Real;
     }
     return;

@@ -182,11 +182,14 @@ struct Stage {
 
 struct FoldGeom;
 FoldGeom fold_geom(const Plan &p);
+// capi.hip: dims the FFT supports (even, dims/2 = 2^a 3^b 5^c)
+bool plan_fft_ok(const Plan &p);
 
 // deposit.hip
 void deposit(Plan &p, double boxsize, int64_t n, const float *pos, const float *masses, double mass,
              bool retain = false);
 void fold_tiles(Plan &p, void *out, int64_t out_rowlen, bool accumulate);
+void fill_zero(void *ptr, size_t bytes, hipStream_t s);
 void tiles_to_field(Plan &p);
 // scan.hip
 void exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, DevBuf &tmp, hipStream_t s);

@@ -111,3 +111,32 @@ def clumped_torch(n, box, dims, n_clumps=100000, seed=4, r_min=0.5, r_max=4.0, d
         out[nu + s:nu + e] = torch.remainder(p, box).float()
     del owner
     return out
+
+
+def gaussian_lattice_torch(n_side, box, dims, sigma_cells, seed=0, device="cuda"):
+    """gaussian_lattice generated on the device (torch's Philox generator): a
+    cubic lattice of n_side^3 points with iid N(0, (sigma_cells*box/dims)^2)
+    offsets per axis, wrapped into [0, box), float32 [n_side^3, 3]."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    ar = torch.arange(n_side, dtype=torch.float64, device=device) * (box / n_side)
+    x, y, z = torch.meshgrid(ar, ar, ar, indexing="ij")
+    p = torch.stack([x.reshape(-1), y.reshape(-1), z.reshape(-1)], 1)
+    p += torch.randn(p.shape, generator=g, dtype=torch.float64, device=device) * (sigma_cells * box / dims)
+    return torch.remainder(p, box).to(torch.float32).contiguous()
+
+
+def uneven_realisations(count=64, seed=100, n_side_range=(64, 256), dims_choices=(128, 256, 384, 512)):
+    """Shapes of a batch of uneven realisations: for r < count, with its own
+    seed (seed + r), n_side = round(exp(U[ln lo, ln hi])) (log-uniform
+    particle counts n_side^3) and dims uniform over dims_choices.
+    Returns [(n_side, dims, seed_r)]."""
+    out = []
+    lo, hi = n_side_range
+    for r in range(count):
+        rng = np.random.default_rng(seed + r)
+        n_side = int(round(float(np.exp(rng.uniform(np.log(lo), np.log(hi))))))
+        d = int(dims_choices[int(rng.integers(len(dims_choices)))])
+        out.append((n_side, d, seed + r))
+    return out

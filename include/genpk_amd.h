@@ -128,6 +128,35 @@ int genpk_adjoint_density(genpk_plan *plan, int nrbins, const double *lambda, do
 int genpk_adjoint_gather(genpk_plan *plan, double boxsize, int64_t n, const float *pos, const float *masses,
                          double mass, float *grad_pos, float *grad_mass);
 
+/* Batched forward (no counterpart in the reference, which runs one
+ * realisation per process): many independent realisations with uneven
+ * particle counts and grid sizes, captured into one hipGraph and launched
+ * with one call.  Jobs run on `branches` parallel graph branches, each with
+ * its own stream and plans.
+ *   genpk_batch_create: stream = the stream launches are ordered on.
+ *   genpk_batch_add: one job = genpk_clear + genpk_deposit + genpk_power of
+ *     a dims^3 plan of `precision`: device pointers pos [n][3], masses [n] or
+ *     NULL; outputs power/keffs double [nrbins], count int [nrbins] (count
+ *     and keffs may be NULL); normalised by total_mass^2.  The buffers must
+ *     stay valid while the batch exists.
+ *   genpk_batch_build: assigns jobs to branches (branch_of_job[i] in
+ *     [0, branches), or NULL for longest-processing-time-first balancing),
+ *     runs every job once (sizing all buffers), captures and instantiates
+ *     the graph.  Synchronises the batch stream first.
+ *   genpk_batch_launch: one graph launch on the batch stream; the outputs are
+ *     complete when that stream is synchronised.  Launches may be repeated
+ *     (e.g. after new positions are written into the same buffers).
+ *   genpk_batch_branch_of: the branch a job was assigned to. */
+typedef struct genpk_batch genpk_batch;
+int genpk_batch_create(genpk_batch **batch, int branches, void *stream);
+int genpk_batch_add(genpk_batch *batch, int64_t dims, int precision, double boxsize, int64_t n, const float *pos,
+                    const float *masses, double mass, int nrbins, double total_mass, double *power, int *count,
+                    double *keffs);
+int genpk_batch_build(genpk_batch *batch, const int *branch_of_job);
+int genpk_batch_launch(genpk_batch *batch);
+int genpk_batch_branch_of(const genpk_batch *batch, int64_t job, int *branch);
+int genpk_batch_destroy(genpk_batch *batch);
+
 /* Stage timing (no counterpart in the reference, which has no timers):
  * while enabled, hipEvents are recorded on the plan's stream around each
  * stage (sort passes, tile deposit, fold, FFT passes, binning, adjoint).
