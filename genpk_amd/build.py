@@ -54,8 +54,12 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     cli_src = os.path.join(HERE, "cli", "gen_pk.cpp")
     if os.path.exists(cli_src):
-        subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(HERE, "..", "include"), cli_src,
-                        "-o", CLI, "-L", HERE, "-lgenpk_amd", "-Wl,-rpath,$ORIGIN"], check=True)
+        rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+        subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(rocm, "include"),
+                        "-I", os.path.join(HERE, "..", "include"), cli_src, "-o", CLI + ".tmp", "-L", HERE,
+                        "-lgenpk_amd", "-L", os.path.join(rocm, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN",
+                        "-Wl,-rpath," + os.path.join(rocm, "lib")], check=True)
+        os.replace(CLI + ".tmp", CLI)
     return LIB
 
 
