@@ -58,10 +58,11 @@ def stage_bytes(name, n, c, p_pad, geo, rec):
         return 2 * rec * n                               # read + write the records
     if name == "deposit_tiles":
         return rec * n + c * K * cells                   # read records, write the halo'd tiles
-    if name == "halo_fold":
-        return 3 * c * K * (cells - prim)                # read + zero the halo cells, update the faces
+    if name in ("fold_y", "fold_x"):
+        rows = 17 if name == "fold_y" else 33            # halo rows of 32 values + a z-halo entry each
+        return 4 * c * K * rows * 33                     # read + zero the sources, read + write the faces
     if name == "fold_fft_z":
-        return c * K * prim + c * p_pad                  # read the primary tile cells, write the padded grid
+        return c * K * prim + c * p_pad                  # read the tile cells (z halos folded on load), write the grid
     if name in ("fft_z", "fft_y", "fft_x", "ifft_z", "ifft_y", "ifft_x", "adjoint_x"):
         return 2 * c * p_pad
     if name == "xpass_bin":

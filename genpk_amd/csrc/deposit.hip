@@ -7,8 +7,10 @@
 //    LDS as 64-bit fixed point (ds_add_u64), then flushes the whole tile with
 //    coalesced stores.  Tiles with more than DEPOSIT_ITEM_MAX particles are
 //    split; their items write FP64 partial tiles that a reduction sums.
-// 3. Fold.  The real-space padded grid [d][d][2(d/2+1)] is assembled by
-//    adding each tile's halo layer onto the neighbouring tile's low face.
+// 3. Fold.  Grid cell values are assembled from the tile store by adding the
+//    halo cells of the previous tiles onto each tile's low faces: y and x
+//    halos in place after the deposit (k_fold_axis), z halos by the readers
+//    (the transform's z pass while loading, export with fold_value).
 //
 // Per-particle index and weight arithmetic follows fieldize.cpp:63-92:
 // u = (double)pos * (d/box); i0 = floor(u) wrapped into [0,d), i1 = i0+1
@@ -175,40 +177,45 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const uint32_t *__restr
   }
 }
 
-// Halo fold: every cell on a tile's low face (local x, y or z = 0) adds the
-// halo cells of the previous tiles that belong to it and zeroes them, so the
-// store holds each grid cell's full value at its own (tile, local) place.
-// Each halo cell maps to exactly one face cell, so the kernel is race free
-// and later accumulating deposits start from zero halos.
-template <typename Real>
-__global__ void __launch_bounds__(256) k_halo_fold(Real *__restrict__ tiles, FoldGeom P) {
+// In-place halo folds along y, then x.  fold_y: one workgroup per tile t adds
+// the y-halo rows (local y = width of the previous tile along y) of that
+// tile into t's rows at local y = 0, for every local x including t's x-halo
+// row, and for the z-halo plane entries of those rows, zeroing the sources.
+// fold_x then adds the previous tile along x's x-halo rows (every local y)
+// into t's rows at local x = 0.  Corner contributions travel through both
+// steps; z halos stay in place for the readers.  Each source row has one
+// destination and sources and destinations never coincide (a width is >= 1),
+// so workgroups do not race; zeroed sources let later deposits accumulate.
+template <typename Real, int AXIS>
+__global__ void __launch_bounds__(256) k_fold_axis(Real *__restrict__ tiles, FoldGeom P) {
   const int64_t t = blockIdx.x;
   const int tz = (int)(t % P.ntz);
   const int ty = (int)((t / P.ntz) % P.nty);
   const int tx = (int)(t / ((int64_t)P.ntz * P.nty));
-  const int wx = min(P.TX, P.d - tx * P.TX), wy = min(P.TY, P.d - ty * P.TY), wz = min(P.TZ, P.d - tz * P.TZ);
-  const int txp = tx == 0 ? P.ntx - 1 : tx - 1, typ = ty == 0 ? P.nty - 1 : ty - 1, tzp = tz == 0 ? P.ntz - 1 : tz - 1;
-  const int hx = min(P.TX, P.d - txp * P.TX), hy = min(P.TY, P.d - typ * P.TY), hz = min(P.TZ, P.d - tzp * P.TZ);
-  // face cells: plane lx = 0 (wy*wz), plane ly = 0 with lx >= 1 ((wx-1)*wz), line-free rest lz = 0 ((wx-1)(wy-1))
-  const int nA = wy * wz, nB = (wx - 1) * wz, nC = (wx - 1) * (wy - 1);
-  for (int f = threadIdx.x; f < nA + nB + nC; f += 256) {
-    int lx, ly, lz;
-    if (f < nA) { lx = 0; ly = f / wz; lz = f - ly * wz; }
-    else if (f < nA + nB) { const int g = f - nA; lx = 1 + g / wz; ly = 0; lz = g - (lx - 1) * wz; }
-    else { const int g = f - nA - nB; lx = 1 + g / (wy - 1); ly = 1 + g - (lx - 1) * (wy - 1); lz = 0; }
-    const int nxs = lx == 0 ? 2 : 1, nys = ly == 0 ? 2 : 1, nzs = lz == 0 ? 2 : 1;
-    double v = 0;
-    for (int a = 0; a < nxs; a++)
-      for (int b = 0; b < nys; b++)
-        for (int c = 0; c < nzs; c++) {
-          if (a == 0 && b == 0 && c == 0) continue;
-          const int64_t st = ((int64_t)(a ? txp : tx) * P.nty + (b ? typ : ty)) * P.ntz + (c ? tzp : tz);
-          const int64_t at = tile_at(P, st, a ? hx : lx, b ? hy : ly, c ? hz : lz);
-          v += (double)tiles[at];
-          tiles[at] = (Real)0;
-        }
-    const int64_t me = tile_at(P, t, lx, ly, lz);
-    tiles[me] = (Real)((double)tiles[me] + v);
+  const int HX = P.TX + 1;
+  int64_t src;
+  int h;
+  if (AXIS == 1) {
+    const int typ = ty == 0 ? P.nty - 1 : ty - 1;
+    h = min(P.TY, P.d - typ * P.TY);
+    src = ((int64_t)tx * P.nty + typ) * P.ntz + tz;
+  } else {
+    const int txp = tx == 0 ? P.ntx - 1 : tx - 1;
+    h = min(P.TX, P.d - txp * P.TX);
+    src = ((int64_t)txp * P.nty + ty) * P.ntz + tz;
+  }
+  // rows: AXIS 1 -> (lx, h) onto (lx, 0), lx < HX; AXIS 0 -> (h, ly) onto (0, ly), ly < HY
+  const int nrows = AXIS == 1 ? HX : P.HY;
+  const int per_row = P.TZ + 1;  // TZ row values, then the z-halo plane entry
+  for (int i = threadIdx.x; i < nrows * per_row; i += 256) {
+    const int r = i / per_row, z = i - r * per_row;
+    const int sl = AXIS == 1 ? r * P.HY + h : h * P.HY + r;   // source row (lx*HY + ly)
+    const int dl = AXIS == 1 ? r * P.HY : r;                    // destination row
+    const int64_t sa = z < P.TZ ? src * P.cm + (int64_t)sl * P.TZ + z : P.zoff + src * P.cz + sl;
+    const int64_t da = z < P.TZ ? t * P.cm + (int64_t)dl * P.TZ + z : P.zoff + t * P.cz + dl;
+    const Real v = tiles[sa];
+    tiles[sa] = (Real)0;
+    tiles[da] = (Real)((double)tiles[da] + (double)v);
   }
 }
 
@@ -220,7 +227,7 @@ __global__ void __launch_bounds__(FOLD_THREADS) k_fold(const Real *__restrict__ 
   const int z = blockIdx.x * FOLD_THREADS + threadIdx.x;
   if (z >= P.d || line >= (int64_t)P.d * P.d) return;
   const int x = (int)(line / P.d), y = (int)(line % P.d);
-  const double v = (double)tiles[cell_at(P, x, y, z)];
+  const double v = fold_value(tiles, P, x, y, z);
   Out *o = out + line * out_rowlen + z;
   *o = accumulate ? (Out)((double)*o + v) : (Out)v;
 }
@@ -269,12 +276,21 @@ void deposit_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass) {
     GENPK_LAUNCH_CHECK();
   }
   {
-    Stage hst(p, "halo_fold");
+    Stage st(p, "fold_y");
     const FoldGeom fg = fold_geom(p);
     if (p.prec == 4)
-      hipLaunchKernelGGL(k_halo_fold<float>, dim3((unsigned)tg.K), dim3(256), 0, s, p.tiles.as<float>(), fg);
+      hipLaunchKernelGGL((k_fold_axis<float, 1>), dim3((unsigned)tg.K), dim3(256), 0, s, p.tiles.as<float>(), fg);
     else
-      hipLaunchKernelGGL(k_halo_fold<double>, dim3((unsigned)tg.K), dim3(256), 0, s, p.tiles.as<double>(), fg);
+      hipLaunchKernelGGL((k_fold_axis<double, 1>), dim3((unsigned)tg.K), dim3(256), 0, s, p.tiles.as<double>(), fg);
+    GENPK_LAUNCH_CHECK();
+  }
+  {
+    Stage st(p, "fold_x");
+    const FoldGeom fg = fold_geom(p);
+    if (p.prec == 4)
+      hipLaunchKernelGGL((k_fold_axis<float, 0>), dim3((unsigned)tg.K), dim3(256), 0, s, p.tiles.as<float>(), fg);
+    else
+      hipLaunchKernelGGL((k_fold_axis<double, 0>), dim3((unsigned)tg.K), dim3(256), 0, s, p.tiles.as<double>(), fg);
     GENPK_LAUNCH_CHECK();
   }
 }

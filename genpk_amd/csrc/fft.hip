@@ -24,7 +24,8 @@ template <> struct FftCfg<double> { static constexpr int ELEMS = 4096; static co
 
 // z pass, forward: rows of d reals -> d/2+1 complex, L rows per workgroup.
 // FROM_TILES: the rows are assembled from the deposit's tile store while
-// loading (the fold), so the real-space grid is never written.
+// loading, halo cells folded in (the fold), so the real-space grid is never
+// written.
 // The LDS footprint (up to 64 KB) allows two workgroups per CU, so the
 // register budget of two waves per SIMD is available.
 template <typename T, bool FROM_TILES>
@@ -42,19 +43,23 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
   const int nl = (int)min((int64_t)L, nrows - row0);
   const int tot = L * M;
   C v[MAXE];
+  T zh[FROM_TILES ? MAXE : 1];
 #pragma unroll
   for (int q = 0; q < MAXE; q++) {
     const int i = threadIdx.x + q * FFT_THREADS;
     v[q] = C{0, 0};
+    if (FROM_TILES) zh[FROM_TILES ? q : 0] = (T)0;
     if (i < tot) {
       const int l = (int)fdM.div((uint32_t)i), m = i - l * M;
       if (l < nl) {
         const int64_t row = row0 + l;
         if (FROM_TILES) {
-          // the tile store is halo-free: both reals sit in one aligned row (TZ even)
+          // both reals sit in one aligned tile row (TZ even); the first cell
+          // of a tile along z adds the previous tile's z-halo cell
           const int x = (int)fdd.div((uint32_t)row), y = (int)(row - (int64_t)x * fg.d);
           const int64_t at = cell_at(fg, x, y, 2 * m);
           v[q] = C{tiles[at], tiles[at + 1]};
+          if (((2 * m) & (fg.TZ - 1)) == 0) zh[FROM_TILES ? q : 0] = tiles[zhalo_at(fg, x, y, 2 * m)];
         } else {
           const T *r = field + row * rowlen;
           v[q] = C{r[2 * m], r[2 * m + 1]};
@@ -65,6 +70,8 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
 #pragma unroll
   for (int q = 0; q < MAXE; q++) {
     const int i = threadIdx.x + q * FFT_THREADS;
+    // z-halo added after all loads are in flight (a + b rounds once, as fold_value)
+    if (FROM_TILES) v[q].x = v[q].x + zh[FROM_TILES ? q : 0];
     if (i < tot) buf[i] = v[q];
   }
   // twiddles of the M-point FFT and of the even/odd split, staged in LDS

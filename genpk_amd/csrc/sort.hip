@@ -46,17 +46,27 @@ struct Src {
   }
 };
 
-// Chunk range of workgroup g.  Pass 1 chunks tile [0, n); pass 2 chunks
-// tile each high-digit segment separately.
+// Chunk range of workgroup g and the place of its digit counts in the
+// count table: counts of digit dg at tb + dg * ts.  Pass 1 chunks tile
+// [0, n) and its table is digit-major over all chunks (tb = g, ts = number
+// of chunks).  Pass 2 chunks tile each high-digit segment separately; its
+// table is segment-major, then digit, then chunk (tb = first chunk of the
+// segment * nlo + chunk within the segment, ts = chunks of the segment), so
+// one exclusive scan of the table gives every chunk's output offsets and
+// every tile's start.
 struct ChunkMap {
   int64_t n;
   int nseg;                    // 0: plain chunks of [0, n)
   const uint32_t *seg_start;   // [nseg+1] record offsets of the segments
   const uint32_t *chunk_start; // [nseg+1] first chunk of each segment
-  __device__ __forceinline__ bool range(int64_t g, int64_t &b, int64_t &e) const {
+  int64_t nchunks;             // pass 1: number of chunks
+  int nlo;                     // pass 2: digits
+  __device__ __forceinline__ bool range(int64_t g, int64_t &b, int64_t &e, int64_t &tb, int64_t &ts) const {
     if (nseg == 0) {
       b = g * SORT_CHUNK;
       e = min(n, b + SORT_CHUNK);
+      tb = g;
+      ts = nchunks;
       return b < e;
     }
     if (g >= (int64_t)chunk_start[nseg]) return false;
@@ -66,8 +76,11 @@ struct ChunkMap {
       if ((int64_t)chunk_start[mid] <= g) lo = mid; else hi = mid;
     }
     const int s = lo;
-    b = (int64_t)seg_start[s] + (g - (int64_t)chunk_start[s]) * SORT_CHUNK;
+    const int64_t c0 = chunk_start[s];
+    b = (int64_t)seg_start[s] + (g - c0) * SORT_CHUNK;
     e = min((int64_t)seg_start[s + 1], b + SORT_CHUNK);
+    tb = c0 * nlo + (g - c0);
+    ts = (int64_t)chunk_start[s + 1] - c0;
     return b < e;
   }
 };
@@ -83,16 +96,14 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// counts[dg * sd + g * sg] = number of records of chunk g with digit dg
+// counts[tb + dg * ts] = number of records of chunk g with digit dg
 template <int NM, int NI>
 __global__ void __launch_bounds__(SORT_THREADS) k_upsweep(Src<NM, NI> src, ChunkMap cm, KeyGeom kg, DigitSel ds,
-                                                          int nb, uint32_t *__restrict__ counts, int64_t sd,
-                                                          int64_t sg) {
+                                                          int nb, uint32_t *__restrict__ counts) {
   constexpr int UNR = 8;
   extern __shared__ uint32_t hist[];
-  int64_t b, e;
-  const uint32_t g = blockIdx.x;
-  if (!cm.range(g, b, e)) return;
+  int64_t b, e, tb, ts;
+  if (!cm.range(blockIdx.x, b, e, tb, ts)) return;
   for (int i = threadIdx.x; i < nb; i += SORT_THREADS) hist[i] = 0;
   __syncthreads();
   for (int64_t base = b; base < e; base += SORT_THREADS * UNR) {
@@ -118,10 +129,10 @@ __global__ void __launch_bounds__(SORT_THREADS) k_upsweep(Src<NM, NI> src, Chunk
     }  }
   __syncthreads();
   for (int i = threadIdx.x; i < nb; i += SORT_THREADS)
-    counts[(int64_t)i * sd + (int64_t)g * sg] = hist[i];
+    counts[tb + (int64_t)i * ts] = hist[i];
 }
 
-// Stable partition of each chunk by digit into dst at offsets[dg*sd + g*sg].
+// Stable partition of each chunk by digit into dst at offsets[tb + dg * ts].
 // Per sub-chunk of SORT_SUB records (each wave owns 1024 consecutive ones):
 // wavefront ballots give every record its rank among equal digits, LDS
 // prefix sums turn ranks into positions of a digit-sorted staging copy, and
@@ -130,8 +141,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_upsweep(Src<NM, NI> src, Chunk
 template <int NM, int NI>
 __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<NM, NI> src, float *__restrict__ dst, ChunkMap cm,
                                                             KeyGeom kg, DigitSel ds, int nb, int dbits,
-                                                            const uint32_t *__restrict__ offsets, int64_t sd,
-                                                            int64_t sg) {
+                                                            const uint32_t *__restrict__ offsets) {
   constexpr int RW = Src<NM, NI>::RW;
   extern __shared__ uint32_t sm[];
   uint32_t *whist = sm;                         // [SORT_WAVES][nb]
@@ -142,12 +152,11 @@ __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<NM, NI> src, flo
   float *stage = reinterpret_cast<float *>(sdig + SORT_SUB);           // [SORT_SUB][RW]
   __shared__ uint32_t wtot[SORT_WAVES];
 
-  int64_t cb, ce;
-  const uint32_t g = blockIdx.x;
-  if (!cm.range(g, cb, ce)) return;
+  int64_t cb, ce, tb, ts;
+  if (!cm.range(blockIdx.x, cb, ce, tb, ts)) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int i = threadIdx.x; i < nb; i += SORT_THREADS)
-    gbase[i] = offsets[(int64_t)i * sd + (int64_t)g * sg];
+    gbase[i] = offsets[tb + (int64_t)i * ts];
   for (int i = threadIdx.x; i < SORT_WAVES * nb; i += SORT_THREADS) whist[i] = 0;
 
   float rec[SORT_PER_LANE][RW];
@@ -276,29 +285,21 @@ __global__ void k_seginfo(const uint32_t *__restrict__ t1scan, int64_t g1, int n
   nchunks[s] = (en - st + SORT_CHUNK - 1) / SORT_CHUNK;
 }
 
-// tot[hi*nlo+lo] = sum over the chunks of segment hi of cnt2[g*nlo+lo]
-__global__ void k_seg_total(const uint32_t *__restrict__ cnt2, const uint32_t *__restrict__ chunk_start, int nhi,
-                            int nlo, uint32_t *__restrict__ tot) {
+// Two-pass tile starts from the scanned pass-2 table: the offset of a tile's
+// first chunk, or the segment start when the segment is empty; tile_start[ks] = n.
+__global__ void k_tile_start_2pass(const uint32_t *__restrict__ t2scan, const uint32_t *__restrict__ chunk_start,
+                                   const uint32_t *__restrict__ seg_start, int nhi, int nlo, int64_t n,
+                                   uint32_t *__restrict__ tile_start) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)nhi * nlo) return;
-  const int hi = (int)(t / nlo), lo = (int)(t % nlo);
-  uint32_t s = 0;
-  for (uint32_t g = chunk_start[hi]; g < chunk_start[hi + 1]; g++) s += cnt2[(int64_t)g * nlo + lo];
-  tot[t] = s;
-}
-
-// cnt2 -> per-chunk output offsets, starting at the tile start
-__global__ void k_seg_offsets(uint32_t *__restrict__ cnt2, const uint32_t *__restrict__ chunk_start, int nhi, int nlo,
-                              const uint32_t *__restrict__ tile_start) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)nhi * nlo) return;
-  const int hi = (int)(t / nlo), lo = (int)(t % nlo);
-  uint32_t run = tile_start[t];
-  for (uint32_t g = chunk_start[hi]; g < chunk_start[hi + 1]; g++) {
-    const uint32_t c = cnt2[(int64_t)g * nlo + lo];
-    cnt2[(int64_t)g * nlo + lo] = run;
-    run += c;
+  const int64_t ks = (int64_t)nhi * nlo;
+  if (t > ks) return;
+  if (t == ks) {
+    tile_start[t] = (uint32_t)n;
+    return;
   }
+  const int hi = (int)(t / nlo), lo = (int)(t % nlo);
+  const int64_t c0 = chunk_start[hi], nc = (int64_t)chunk_start[hi + 1] - c0;
+  tile_start[t] = nc ? t2scan[c0 * nlo + lo * nc] : seg_start[hi];
 }
 
 // Single-pass tile starts: tile_start[k] = t1scan[k*g1] (chunk 0), tile_start[ks] = n
@@ -307,10 +308,6 @@ __global__ void k_tile_start_1pass(const uint32_t *__restrict__ t1scan, int64_t 
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k > ks) return;
   tile_start[k] = (k == ks) ? (uint32_t)n : t1scan[(int64_t)k * g1];
-}
-
-__global__ void k_set_last(uint32_t *__restrict__ a, int64_t at, uint32_t v) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) a[at] = v;
 }
 
 // Work items per tile.  store mode: every real tile gets at least one item
@@ -362,18 +359,17 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     ks = nb;
     uint32_t *t1 = static_cast<uint32_t *>(p.table1.ensure((size_t)nb * g1 * 4));
     DigitSel dsel{0, (uint32_t)(nb - 1)};
-    ChunkMap cm{n, 0, nullptr, nullptr};
+    ChunkMap cm{n, 0, nullptr, nullptr, g1, 0};
     {
       Stage st(p, "sort_upsweep1");
-      hipLaunchKernelGGL((k_upsweep<NM, NI>), dim3((unsigned)g1), dim3(SORT_THREADS), nb * 4, s, raw, cm, kg, dsel, nb, t1,
-                         g1, (int64_t)1);
+      hipLaunchKernelGGL((k_upsweep<NM, NI>), dim3((unsigned)g1), dim3(SORT_THREADS), nb * 4, s, raw, cm, kg, dsel, nb, t1);
       GENPK_LAUNCH_CHECK();
     }
     exclusive_scan_u32(t1, t1, (int64_t)nb * g1, p.scan_tmp, s);
     {
       Stage st(p, "sort_downsweep1");
       hipLaunchKernelGGL((k_downsweep<NM, NI>), dim3((unsigned)g1), dim3(SORT_THREADS), downsweep_lds(nb, RW), s, raw, A,
-                         cm, kg, dsel, nb, tg.hi_bits, t1, g1, (int64_t)1);
+                         cm, kg, dsel, nb, tg.hi_bits, t1);
       GENPK_LAUNCH_CHECK();
     }
     tile_start = static_cast<uint32_t *>(p.tile_start.ensure((size_t)(ks + 1) * 4));
@@ -387,18 +383,18 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     // pass 1: high digit over [0, n)
     uint32_t *t1 = static_cast<uint32_t *>(p.table1.ensure((size_t)nhi * g1 * 4));
     DigitSel d1{tg.lo_bits, (uint32_t)(nhi - 1)};
-    ChunkMap cm1{n, 0, nullptr, nullptr};
+    ChunkMap cm1{n, 0, nullptr, nullptr, g1, 0};
     {
       Stage st(p, "sort_upsweep1");
       hipLaunchKernelGGL((k_upsweep<NM, NI>), dim3((unsigned)g1), dim3(SORT_THREADS), nhi * 4, s, raw, cm1, kg, d1, nhi,
-                         t1, g1, (int64_t)1);
+                         t1);
       GENPK_LAUNCH_CHECK();
     }
     exclusive_scan_u32(t1, t1, (int64_t)nhi * g1, p.scan_tmp, s);
     {
       Stage st(p, "sort_downsweep1");
       hipLaunchKernelGGL((k_downsweep<NM, NI>), dim3((unsigned)g1), dim3(SORT_THREADS), downsweep_lds(nhi, RW), s, raw, A,
-                         cm1, kg, d1, nhi, tg.hi_bits, t1, g1, (int64_t)1);
+                         cm1, kg, d1, nhi, tg.hi_bits, t1);
       GENPK_LAUNCH_CHECK();
     }
     // segments of pass 2
@@ -409,31 +405,27 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     GENPK_LAUNCH_CHECK();
     exclusive_scan_u32(nchunks, chunk_start, nhi + 1, p.scan_tmp, s);
     const int64_t g2 = g1 + nhi;  // upper bound of pass-2 chunks
-    uint32_t *t2 = static_cast<uint32_t *>(p.table2.ensure((size_t)g2 * nlo * 4 + (size_t)(ks + 1) * 4 * 2));
-    uint32_t *tot = t2 + g2 * nlo;
+    uint32_t *t2 = static_cast<uint32_t *>(p.table2.ensure((size_t)g2 * nlo * 4));
     DigitSel d2{0, (uint32_t)(nlo - 1)};
-    ChunkMap cm2{n, nhi, seg_start, chunk_start};
+    ChunkMap cm2{n, nhi, seg_start, chunk_start, 0, nlo};
     Src<NM, NI> pa{A, nullptr, false};
     {
       Stage st(p, "sort_upsweep2");
-      hipLaunchKernelGGL((k_upsweep<NM, NI>), dim3((unsigned)g2), dim3(SORT_THREADS), nlo * 4, s, pa, cm2, kg, d2, nlo, t2,
-                         (int64_t)1, (int64_t)nlo);
+      hipLaunchKernelGGL((k_upsweep<NM, NI>), dim3((unsigned)g2), dim3(SORT_THREADS), nlo * 4, s, pa, cm2, kg, d2, nlo,
+                         t2);
       GENPK_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_seg_total, dim3((unsigned)ceil_div(ks, 256)), dim3(256), 0, s, t2, chunk_start, nhi, nlo,
-                       tot);
-    GENPK_LAUNCH_CHECK();
+    // the written part of the table is its first (pass-2 chunks) * nlo entries;
+    // the rest, scanned along, is never read
+    exclusive_scan_u32(t2, t2, g2 * nlo, p.scan_tmp, s);
     tile_start = static_cast<uint32_t *>(p.tile_start.ensure((size_t)(ks + 1) * 4));
-    hipLaunchKernelGGL(k_set_last, dim3(1), dim3(1), 0, s, tot, ks, 0u);
-    GENPK_LAUNCH_CHECK();
-    exclusive_scan_u32(tot, tile_start, ks + 1, p.scan_tmp, s);
-    hipLaunchKernelGGL(k_seg_offsets, dim3((unsigned)ceil_div(ks, 256)), dim3(256), 0, s, t2, chunk_start, nhi, nlo,
-                       tile_start);
+    hipLaunchKernelGGL(k_tile_start_2pass, dim3((unsigned)ceil_div(ks + 1, 256)), dim3(256), 0, s, t2, chunk_start,
+                       seg_start, nhi, nlo, n, tile_start);
     GENPK_LAUNCH_CHECK();
     {
       Stage st(p, "sort_downsweep2");
       hipLaunchKernelGGL((k_downsweep<NM, NI>), dim3((unsigned)g2), dim3(SORT_THREADS), downsweep_lds(nlo, RW), s, pa, B,
-                         cm2, kg, d2, nlo, tg.lo_bits, t2, (int64_t)1, (int64_t)nlo);
+                         cm2, kg, d2, nlo, tg.lo_bits, t2);
       GENPK_LAUNCH_CHECK();
     }
     sorted = B;
