@@ -11,11 +11,15 @@ namespace genpk {
 
 namespace {
 
-constexpr int SORT_THREADS = 256;
+// 8 waves of 8 records per lane per sub-chunk: the LDS staging of a
+// 4096-record sub-chunk (~67 KB with 256 digits) leaves room for two
+// workgroups (16 waves) per CU, which measured faster at 1024^3 than 4 waves
+// x 16 records (8.3 -> 7.2 ms per downsweep) or 16 waves x 4 records.
+constexpr int SORT_THREADS = 512;
 constexpr int SORT_WAVES = SORT_THREADS / 64;
-constexpr int SORT_PER_LANE = 16;                                   // iterations per wave per sub-chunk
+constexpr int SORT_PER_LANE = 8;                                    // iterations per wave per sub-chunk
 constexpr int SORT_SUB = SORT_THREADS * SORT_PER_LANE;              // 4096 records per sub-chunk
-constexpr int SORT_CHUNK = 8 * SORT_SUB;                            // 32768 records per workgroup
+constexpr int SORT_CHUNK = 16 * SORT_SUB;                           // 65536 records per workgroup
 
 // Source of records: either the caller's arrays (positions [n][3], optional
 // masses [n]; the index payload is the record's position in [0, n)) or a
