@@ -11,15 +11,14 @@ namespace genpk {
 
 namespace {
 
-// 8 waves of 8 records per lane per sub-chunk: the LDS staging of a
-// 4096-record sub-chunk (~67 KB with 256 digits) leaves room for two
-// workgroups (16 waves) per CU, which measured faster at 1024^3 than 4 waves
-// x 16 records (8.3 -> 7.2 ms per downsweep) or 16 waves x 4 records.
+// 8 waves per workgroup.  The downsweep's records per lane per sub-chunk (PL)
+// is the largest of 8, 4, 2 whose LDS staging leaves room for two workgroups
+// (16 waves) per CU: at 1024^3 this measured faster than 4 waves x 16 records
+// (8.3 -> 7.2 ms per downsweep) or 16 waves x 4 records, and one workgroup
+// per CU (16-byte records staged 4096 at a time) doubles the time.
 constexpr int SORT_THREADS = 512;
 constexpr int SORT_WAVES = SORT_THREADS / 64;
-constexpr int SORT_PER_LANE = 8;                                    // iterations per wave per sub-chunk
-constexpr int SORT_SUB = SORT_THREADS * SORT_PER_LANE;              // 4096 records per sub-chunk
-constexpr int SORT_CHUNK = 16 * SORT_SUB;                           // 65536 records per workgroup
+constexpr int SORT_CHUNK = 65536;                                   // records per workgroup
 
 // Source of records: either the caller's arrays (positions [n][3], optional
 // masses [n]; the index payload is the record's position in [0, n)) or a
@@ -137,23 +136,24 @@ __global__ void __launch_bounds__(SORT_THREADS) k_upsweep(Src<NM, NI> src, Chunk
 }
 
 // Stable partition of each chunk by digit into dst at offsets[tb + dg * ts].
-// Per sub-chunk of SORT_SUB records (each wave owns 1024 consecutive ones):
+// Per sub-chunk of SORT_THREADS * PL records (each wave owns 64 * PL consecutive ones):
 // wavefront ballots give every record its rank among equal digits, LDS
 // prefix sums turn ranks into positions of a digit-sorted staging copy, and
 // the staging copy is written out as contiguous runs.  The next sub-chunk's
 // records are loaded while the current one is written out.
-template <int NM, int NI>
+template <int NM, int NI, int PL>
 __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<NM, NI> src, float *__restrict__ dst, ChunkMap cm,
                                                             KeyGeom kg, DigitSel ds, int nb, int dbits,
                                                             const uint32_t *__restrict__ offsets) {
   constexpr int RW = Src<NM, NI>::RW;
+  constexpr int SUBN = SORT_THREADS * PL;  // records per sub-chunk
   extern __shared__ uint32_t sm[];
   uint32_t *whist = sm;                         // [SORT_WAVES][nb]
   uint32_t *lstart = whist + SORT_WAVES * nb;   // [nb]
   uint32_t *gbase = lstart + nb;                // [nb]
   uint32_t *subcnt = gbase + nb;                // [nb]
-  uint16_t *sdig = reinterpret_cast<uint16_t *>(subcnt + nb);          // [SORT_SUB]
-  float *stage = reinterpret_cast<float *>(sdig + SORT_SUB);           // [SORT_SUB][RW]
+  uint16_t *sdig = reinterpret_cast<uint16_t *>(subcnt + nb);          // [SUBN]
+  float *stage = reinterpret_cast<float *>(sdig + SUBN);           // [SUBN][RW]
   __shared__ uint32_t wtot[SORT_WAVES];
 
   int64_t cb, ce, tb, ts;
@@ -163,21 +163,21 @@ __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<NM, NI> src, flo
     gbase[i] = offsets[tb + (int64_t)i * ts];
   for (int i = threadIdx.x; i < SORT_WAVES * nb; i += SORT_THREADS) whist[i] = 0;
 
-  float rec[SORT_PER_LANE][RW];
-  const int wbase = wave * (SORT_PER_LANE * 64) + lane;
+  float rec[PL][RW];
+  const int wbase = wave * (PL * 64) + lane;
 #pragma unroll
-  for (int it = 0; it < SORT_PER_LANE; it++) {
+  for (int it = 0; it < PL; it++) {
     const int64_t i = cb + wbase + it * 64;
     if (i < ce) src.load(i, rec[it]);
   }
   __syncthreads();
 
-  for (int64_t s0 = cb; s0 < ce; s0 += SORT_SUB) {
-    const int nsub = (int)min((int64_t)SORT_SUB, ce - s0);
-    uint32_t dig[SORT_PER_LANE];
-    uint32_t rnk[SORT_PER_LANE];
+  for (int64_t s0 = cb; s0 < ce; s0 += SUBN) {
+    const int nsub = (int)min((int64_t)SUBN, ce - s0);
+    uint32_t dig[PL];
+    uint32_t rnk[PL];
 #pragma unroll
-    for (int it = 0; it < SORT_PER_LANE; it++) {
+    for (int it = 0; it < PL; it++) {
       const int loc = wbase + it * 64;
       const bool valid = loc < nsub;
       const uint32_t dg = valid ? ds.of(tile_key(rec[it], kg)) : 0u;
@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<NM, NI> src, flo
     }
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < SORT_PER_LANE; it++) {
+    for (int it = 0; it < PL; it++) {
       const int loc = wbase + it * 64;
       if (loc < nsub) {
         const uint32_t dg = dig[it];
@@ -253,9 +253,9 @@ __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<NM, NI> src, flo
     __syncthreads();
     // prefetch the next sub-chunk while this one is written out
     {
-      const int64_t n0 = s0 + SORT_SUB;
+      const int64_t n0 = s0 + SUBN;
 #pragma unroll
-      for (int it = 0; it < SORT_PER_LANE; it++) {
+      for (int it = 0; it < PL; it++) {
         const int64_t i = n0 + wbase + it * 64;
         if (i < ce) src.load(i, rec[it]);
       }
@@ -340,8 +340,33 @@ __global__ void k_heavy_list(const uint32_t *__restrict__ pcnt, int64_t ks, uint
   if (pcnt[k] > 1) heavy[atomicAdd(nheavy, 1u)] = (uint32_t)k;
 }
 
-size_t downsweep_lds(int nb, int rw) {
-  return (size_t)(SORT_WAVES * nb + 3 * nb) * 4 + (size_t)SORT_SUB * 2 + (size_t)SORT_SUB * rw * 4;
+size_t downsweep_lds(int nb, int rw, int pl) {
+  const size_t sub = (size_t)SORT_THREADS * pl;
+  return (size_t)(SORT_WAVES * nb + 3 * nb) * 4 + sub * 2 + sub * rw * 4;
+}
+
+int downsweep_pl(int nb, int rw) {
+  for (int pl : {8, 4})
+    if (downsweep_lds(nb, rw, pl) <= 80 * 1024) return pl;
+  return 2;
+}
+
+template <int NM, int NI>
+void launch_downsweep(int64_t grid, hipStream_t s, Src<NM, NI> src, float *dst, const ChunkMap &cm, const KeyGeom &kg,
+                      const DigitSel &ds, int nb, int dbits, const uint32_t *offsets) {
+  constexpr int RW = Src<NM, NI>::RW;
+  const int pl = downsweep_pl(nb, RW);
+  const size_t lds = downsweep_lds(nb, RW, pl);
+  if (pl == 8)
+    hipLaunchKernelGGL((k_downsweep<NM, NI, 8>), dim3((unsigned)grid), dim3(SORT_THREADS), lds, s, src, dst, cm, kg, ds,
+                       nb, dbits, offsets);
+  else if (pl == 4)
+    hipLaunchKernelGGL((k_downsweep<NM, NI, 4>), dim3((unsigned)grid), dim3(SORT_THREADS), lds, s, src, dst, cm, kg, ds,
+                       nb, dbits, offsets);
+  else
+    hipLaunchKernelGGL((k_downsweep<NM, NI, 2>), dim3((unsigned)grid), dim3(SORT_THREADS), lds, s, src, dst, cm, kg, ds,
+                       nb, dbits, offsets);
+  GENPK_LAUNCH_CHECK();
 }
 
 template <int NM, int NI>
@@ -372,9 +397,7 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     exclusive_scan_u32(t1, t1, (int64_t)nb * g1, p.scan_tmp, s);
     {
       Stage st(p, "sort_downsweep1");
-      hipLaunchKernelGGL((k_downsweep<NM, NI>), dim3((unsigned)g1), dim3(SORT_THREADS), downsweep_lds(nb, RW), s, raw, A,
-                         cm, kg, dsel, nb, tg.hi_bits, t1);
-      GENPK_LAUNCH_CHECK();
+      launch_downsweep<NM, NI>(g1, s, raw, A, cm, kg, dsel, nb, tg.hi_bits, t1);
     }
     tile_start = static_cast<uint32_t *>(p.tile_start.ensure((size_t)(ks + 1) * 4));
     hipLaunchKernelGGL(k_tile_start_1pass, dim3((unsigned)ceil_div(ks + 1, 256)), dim3(256), 0, s, t1, g1, (int)ks,
@@ -397,9 +420,7 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     exclusive_scan_u32(t1, t1, (int64_t)nhi * g1, p.scan_tmp, s);
     {
       Stage st(p, "sort_downsweep1");
-      hipLaunchKernelGGL((k_downsweep<NM, NI>), dim3((unsigned)g1), dim3(SORT_THREADS), downsweep_lds(nhi, RW), s, raw, A,
-                         cm1, kg, d1, nhi, tg.hi_bits, t1);
-      GENPK_LAUNCH_CHECK();
+      launch_downsweep<NM, NI>(g1, s, raw, A, cm1, kg, d1, nhi, tg.hi_bits, t1);
     }
     // segments of pass 2
     uint32_t *seg = static_cast<uint32_t *>(p.seginfo.ensure((size_t)(nhi + 1) * 3 * 4));
@@ -428,9 +449,7 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     GENPK_LAUNCH_CHECK();
     {
       Stage st(p, "sort_downsweep2");
-      hipLaunchKernelGGL((k_downsweep<NM, NI>), dim3((unsigned)g2), dim3(SORT_THREADS), downsweep_lds(nlo, RW), s, pa, B,
-                         cm2, kg, d2, nlo, tg.lo_bits, t2);
-      GENPK_LAUNCH_CHECK();
+      launch_downsweep<NM, NI>(g2, s, pa, B, cm2, kg, d2, nlo, tg.lo_bits, t2);
     }
     sorted = B;
   }
@@ -475,10 +494,6 @@ KeyGeom key_geom(const Plan &p, double boxsize) {
   g.sx = p.tg.sx; g.sy = p.tg.sy; g.sz = p.tg.sz;
   g.ntx = p.tg.ntx; g.nty = p.tg.nty; g.ntz = p.tg.ntz;
   return g;
-}
-
-size_t downsweep_lds(int nb, int rw) {
-  return (size_t)(SORT_WAVES * nb + 3 * nb) * 4 + (size_t)SORT_SUB * 2 + (size_t)SORT_SUB * rw * 4;
 }
 
 TileOrder sort_by_tile(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, const float *masses, bool with_index,
