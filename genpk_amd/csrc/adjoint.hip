@@ -16,8 +16,9 @@
 // density (z and y passes first).  In the last two cases the x pass runs
 // forward FFT -> weight -> inverse FFT on each column block in LDS, so the
 // spectrum is never written.  The gather visits the particles in tile order
-// (the deposit's partition, with each record's index), reads dL/drho from a
-// halo'd tile staged in LDS, and writes the gradients at the records' indices.
+// (the deposit's partition), reads dL/drho from a halo'd tile staged in LDS
+// and writes the gradients in tile order; the sort's permutations then take
+// them back to the particles' order.
 #include "fft_core.h"
 #include "tilesort.h"
 
@@ -138,14 +139,16 @@ struct GatherParams {
 };
 
 // One workgroup per work item of the tile order: stage the halo'd tile of
-// G = dL/drho in LDS, then the gradients of the item's records.
-template <typename T, int NM>
+// G = dL/drho in LDS, then the gradients of the item's records, written at
+// the records' own (tile-order) positions as (dL/dx, dL/dy, dL/dz[, dL/dm]):
+// coalesced.
+template <typename T, int NM, int GM>
 __global__ void __launch_bounds__(GATHER_THREADS) k_gather_tiles(const T *__restrict__ G, const float *__restrict__ rec,
                                                                  const uint32_t *__restrict__ tile_start,
                                                                  const uint32_t *__restrict__ item_start,
-                                                                 GatherParams P, float *__restrict__ gpos,
-                                                                 float *__restrict__ gmass) {
-  constexpr int RW = 4 + NM;
+                                                                 GatherParams P, float *__restrict__ out) {
+  constexpr int RW = 3 + NM;
+  constexpr int RWO = 3 + GM;
   extern __shared__ unsigned char lds_raw[];
   T *g = reinterpret_cast<T *>(lds_raw);
   ItemRange ir;
@@ -198,11 +201,30 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather_tiles(const T *__rest
     const double gm = b00 * a00 + b10 * a10 + b01 * a01 + b11 * a11;
     const double m = NM ? (double)r[3] : P.mass;
     const double s = m * units;
-    const uint32_t idx = __float_as_uint(r[RW - 1]);
-    gpos[3 * (int64_t)idx + 0] = (float)(dgx * s);
-    gpos[3 * (int64_t)idx + 1] = (float)(dgy * s);
-    gpos[3 * (int64_t)idx + 2] = (float)(dgz * s);
-    if (gmass) gmass[idx] = (float)gm;
+    float *o = out + (int64_t)i * RWO;
+    o[0] = (float)(dgx * s);
+    o[1] = (float)(dgy * s);
+    o[2] = (float)(dgz * s);
+    if (GM) o[3] = (float)gm;
+  }
+}
+
+// Gradients back to the particles' input order: particle j sits at sorted
+// position perm2[perm1[j]] (perm1 alone after a one-pass sort).  Reads of
+// perm1 and writes are coalesced; the two lookups are random reads, which
+// cost far less than scattered partial-line writes.
+template <int RWO, int TWO>
+__global__ void k_unpermute(const float *__restrict__ rec, const uint32_t *__restrict__ perm1,
+                            const uint32_t *__restrict__ perm2, int64_t n, float *__restrict__ gpos,
+                            float *__restrict__ gmass) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    int64_t at = perm1[j];
+    if (TWO) at = perm2[at];
+    const float *r = rec + at * RWO;
+    gpos[3 * j + 0] = r[0];
+    gpos[3 * j + 1] = r[1];
+    gpos[3 * j + 2] = r[2];
+    if (RWO == 4) gmass[j] = r[3];
   }
 }
 
@@ -231,8 +253,8 @@ void adjoint_x_impl(Plan &p, const AdjWeight &w) {
   GENPK_LAUNCH_CHECK();
 }
 
-template <typename T, int NM>
-void gather_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass, float *grad_pos, float *grad_mass) {
+template <typename T, int NM, int GM>
+void gather_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass, float *out) {
   const TileGeom &tg = p.tg;
   GatherParams P;
   P.kg = kg;
@@ -244,9 +266,9 @@ void gather_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass, fl
   P.mass = mass;
   P.fhz = FastDiv((uint32_t)tg.HZ);
   P.fhy = FastDiv((uint32_t)tg.HY);
-  hipLaunchKernelGGL((k_gather_tiles<T, NM>), dim3((unsigned)o.max_items), dim3(GATHER_THREADS),
+  hipLaunchKernelGGL((k_gather_tiles<T, NM, GM>), dim3((unsigned)o.max_items), dim3(GATHER_THREADS),
                      (size_t)tg.cells * sizeof(T), p.stream, p.field.as<const T>(), o.rec, o.tile_start, o.item_start,
-                     P, grad_pos, grad_mass);
+                     P, out);
   GENPK_LAUNCH_CHECK();
 }
 
@@ -296,7 +318,7 @@ void adjoint_gather(Plan &p, double boxsize, int64_t n, const float *pos, const 
   const int64_t cap = (int64_t)1 << 31;
   const RetainedOrder &r = p.order;
   const bool reuse = r.valid && r.pos == pos && r.masses == masses && r.n == n && r.boxsize == boxsize &&
-                     r.to.has_index;
+                     r.to.has_perm;
   const KeyGeom kg = key_geom(p, boxsize);
   int64_t done = 0;
   do {
@@ -316,11 +338,36 @@ void adjoint_gather(Plan &p, double boxsize, int64_t n, const float *pos, const 
     }
     float *gp = grad_pos + 3 * done;
     float *gmp = grad_mass ? grad_mass + done : nullptr;
-    Stage st(p, "adjoint_gather");
-    if (p.prec == 4) {
-      if (masses) gather_impl<float, 1>(p, kg, o, mass, gp, gmp); else gather_impl<float, 0>(p, kg, o, mass, gp, gmp);
-    } else {
-      if (masses) gather_impl<double, 1>(p, kg, o, mass, gp, gmp); else gather_impl<double, 0>(p, kg, o, mass, gp, gmp);
+    // output records go to the record buffer that does not hold the sorted records
+    float *out = (o.rec == p.recA.as<float>()) ? p.recB.as<float>() : p.recA.as<float>();
+    const int gm = grad_mass ? 1 : 0;
+    {
+      Stage st(p, "adjoint_gather");
+      if (p.prec == 4) {
+        if (masses)
+          gm ? gather_impl<float, 1, 1>(p, kg, o, mass, out) : gather_impl<float, 1, 0>(p, kg, o, mass, out);
+        else
+          gm ? gather_impl<float, 0, 1>(p, kg, o, mass, out) : gather_impl<float, 0, 0>(p, kg, o, mass, out);
+      } else {
+        if (masses)
+          gm ? gather_impl<double, 1, 1>(p, kg, o, mass, out) : gather_impl<double, 1, 0>(p, kg, o, mass, out);
+        else
+          gm ? gather_impl<double, 0, 1>(p, kg, o, mass, out) : gather_impl<double, 0, 0>(p, kg, o, mass, out);
+      }
+    }
+    {
+      Stage st(p, "adjoint_unpermute");
+      const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(m, 256), 1 << 22);
+      const bool two = o.perm2 != nullptr;
+      if (gm && two)
+        hipLaunchKernelGGL((k_unpermute<4, 1>), dim3(grid), dim3(256), 0, p.stream, out, o.perm1, o.perm2, m, gp, gmp);
+      else if (gm)
+        hipLaunchKernelGGL((k_unpermute<4, 0>), dim3(grid), dim3(256), 0, p.stream, out, o.perm1, o.perm2, m, gp, gmp);
+      else if (two)
+        hipLaunchKernelGGL((k_unpermute<3, 1>), dim3(grid), dim3(256), 0, p.stream, out, o.perm1, o.perm2, m, gp, gmp);
+      else
+        hipLaunchKernelGGL((k_unpermute<3, 0>), dim3(grid), dim3(256), 0, p.stream, out, o.perm1, o.perm2, m, gp, gmp);
+      GENPK_LAUNCH_CHECK();
     }
     done += m;
   } while (done < n);

@@ -47,7 +47,7 @@ struct DepParams {
 };
 
 // Records are RW floats: x, y, z, then the mass when NM (FP64 LDS
-// accumulation), then an index payload that the deposit ignores.
+// accumulation).
 template <typename Real, int NM, int RW>
 __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict__ rec,
                                                          const uint32_t *__restrict__ tile_start,
@@ -345,9 +345,9 @@ void deposit(Plan &p, double boxsize, int64_t n, const float *pos, const float *
     const float *mp = masses ? masses + done : nullptr;
     const TileOrder o = sort_by_tile(p, kg, m, pos + 3 * done, mp, keep, !p.tiles_live);
     if (masses)
-      keep ? deposit_impl<1, 5>(p, kg, o, mass) : deposit_impl<1, 4>(p, kg, o, mass);
+      deposit_impl<1, 4>(p, kg, o, mass);
     else
-      keep ? deposit_impl<0, 4>(p, kg, o, mass) : deposit_impl<0, 3>(p, kg, o, mass);
+      deposit_impl<0, 3>(p, kg, o, mass);
     p.tiles_live = true;
     if (keep) {
       p.order.valid = true;

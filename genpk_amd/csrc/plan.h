@@ -83,11 +83,13 @@ struct BinTables {
 enum class FieldState { None, Real, Spectrum, YDone };
 
 // Particles partitioned by tile (plan scratch; see sort_by_tile): records
-// of rw floats (x, y, z[, mass][, index]), tile starts, work items.
+// of rw floats (x, y, z[, mass]), tile starts, work items, and optionally
+// the permutations input -> sorted position.
 struct TileOrder {
   const float *rec = nullptr;
   int rw = 0;
-  bool has_mass = false, has_index = false;
+  bool has_mass = false, has_perm = false;
+  const uint32_t *perm1 = nullptr, *perm2 = nullptr;
   const uint32_t *tile_start = nullptr, *item_start = nullptr, *part_start = nullptr;
   const uint32_t *nheavy = nullptr, *heavy = nullptr;
   int64_t ks = 0, n = 0, max_items = 0;
@@ -146,6 +148,7 @@ struct Plan {
 
   // sort / deposit scratch
   DevBuf recA, recB, table1, table2, seginfo, tile_start, item_start, part_start, heavy, partials;
+  DevBuf permA, permB;      // sort permutations kept for the adjoint gather
   DevBuf scan_tmp;
   RetainedOrder order;
 

@@ -21,11 +21,10 @@ constexpr int SORT_WAVES = SORT_THREADS / 64;
 constexpr int SORT_CHUNK = 65536;                                   // records per workgroup
 
 // Source of records: either the caller's arrays (positions [n][3], optional
-// masses [n]; the index payload is the record's position in [0, n)) or a
-// packed record buffer of RW floats per record.
-template <int NM, int NI>
+// masses [n]) or a packed record buffer of RW floats per record.
+template <int NM>
 struct Src {
-  static constexpr int RW = 3 + NM + NI;
+  static constexpr int RW = 3 + NM;
   const float *pos;     // raw positions (stride 3) or packed records (stride RW)
   const float *masses;  // raw masses when reading the caller's arrays
   bool raw;
@@ -41,7 +40,6 @@ struct Src {
       r[1] = pos[3 * i + 1];
       r[2] = pos[3 * i + 2];
       if (NM) r[3] = masses[i];
-      if (NI) r[3 + NM] = __uint_as_float((uint32_t)i);
     } else {
 #pragma unroll
       for (int k = 0; k < RW; k++) r[k] = pos[RW * i + k];
@@ -100,8 +98,8 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 }
 
 // counts[tb + dg * ts] = number of records of chunk g with digit dg
-template <int NM, int NI>
-__global__ void __launch_bounds__(SORT_THREADS) k_upsweep(Src<NM, NI> src, ChunkMap cm, KeyGeom kg, DigitSel ds,
+template <int NM>
+__global__ void __launch_bounds__(SORT_THREADS) k_upsweep(Src<NM> src, ChunkMap cm, KeyGeom kg, DigitSel ds,
                                                           int nb, uint32_t *__restrict__ counts) {
   constexpr int UNR = 8;
   extern __shared__ uint32_t hist[];
@@ -136,16 +134,18 @@ __global__ void __launch_bounds__(SORT_THREADS) k_upsweep(Src<NM, NI> src, Chunk
 }
 
 // Stable partition of each chunk by digit into dst at offsets[tb + dg * ts].
+// perm (optional): perm[i] = output position of input record i.
 // Per sub-chunk of SORT_THREADS * PL records (each wave owns 64 * PL consecutive ones):
 // wavefront ballots give every record its rank among equal digits, LDS
 // prefix sums turn ranks into positions of a digit-sorted staging copy, and
 // the staging copy is written out as contiguous runs.  The next sub-chunk's
 // records are loaded while the current one is written out.
-template <int NM, int NI, int PL>
-__global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<NM, NI> src, float *__restrict__ dst, ChunkMap cm,
+template <int NM, int PL>
+__global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<NM> src, float *__restrict__ dst, ChunkMap cm,
                                                             KeyGeom kg, DigitSel ds, int nb, int dbits,
-                                                            const uint32_t *__restrict__ offsets) {
-  constexpr int RW = Src<NM, NI>::RW;
+                                                            const uint32_t *__restrict__ offsets,
+                                                            uint32_t *__restrict__ perm) {
+  constexpr int RW = Src<NM>::RW;
   constexpr int SUBN = SORT_THREADS * PL;  // records per sub-chunk
   extern __shared__ uint32_t sm[];
   uint32_t *whist = sm;                         // [SORT_WAVES][nb]
@@ -244,10 +244,12 @@ __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<NM, NI> src, flo
       const int loc = wbase + it * 64;
       if (loc < nsub) {
         const uint32_t dg = dig[it];
-        const uint32_t p = lstart[dg] + whist[wave * nb + dg] + rnk[it];
+        const uint32_t within = whist[wave * nb + dg] + rnk[it];  // place in the digit's run of this sub-chunk
+        const uint32_t p = lstart[dg] + within;
 #pragma unroll
         for (int k = 0; k < RW; k++) stage[p * RW + k] = rec[it][k];
         sdig[p] = (uint16_t)dg;
+        if (perm) perm[s0 + loc] = gbase[dg] + within;  // where this input record goes (coalesced)
       }
     }
     __syncthreads();
@@ -351,34 +353,36 @@ int downsweep_pl(int nb, int rw) {
   return 2;
 }
 
-template <int NM, int NI>
-void launch_downsweep(int64_t grid, hipStream_t s, Src<NM, NI> src, float *dst, const ChunkMap &cm, const KeyGeom &kg,
-                      const DigitSel &ds, int nb, int dbits, const uint32_t *offsets) {
-  constexpr int RW = Src<NM, NI>::RW;
+template <int NM>
+void launch_downsweep(int64_t grid, hipStream_t s, Src<NM> src, float *dst, const ChunkMap &cm, const KeyGeom &kg,
+                      const DigitSel &ds, int nb, int dbits, const uint32_t *offsets, uint32_t *perm) {
+  constexpr int RW = Src<NM>::RW;
   const int pl = downsweep_pl(nb, RW);
   const size_t lds = downsweep_lds(nb, RW, pl);
   if (pl == 8)
-    hipLaunchKernelGGL((k_downsweep<NM, NI, 8>), dim3((unsigned)grid), dim3(SORT_THREADS), lds, s, src, dst, cm, kg, ds,
-                       nb, dbits, offsets);
+    hipLaunchKernelGGL((k_downsweep<NM, 8>), dim3((unsigned)grid), dim3(SORT_THREADS), lds, s, src, dst, cm, kg, ds,
+                       nb, dbits, offsets, perm);
   else if (pl == 4)
-    hipLaunchKernelGGL((k_downsweep<NM, NI, 4>), dim3((unsigned)grid), dim3(SORT_THREADS), lds, s, src, dst, cm, kg, ds,
-                       nb, dbits, offsets);
+    hipLaunchKernelGGL((k_downsweep<NM, 4>), dim3((unsigned)grid), dim3(SORT_THREADS), lds, s, src, dst, cm, kg, ds,
+                       nb, dbits, offsets, perm);
   else
-    hipLaunchKernelGGL((k_downsweep<NM, NI, 2>), dim3((unsigned)grid), dim3(SORT_THREADS), lds, s, src, dst, cm, kg, ds,
-                       nb, dbits, offsets);
+    hipLaunchKernelGGL((k_downsweep<NM, 2>), dim3((unsigned)grid), dim3(SORT_THREADS), lds, s, src, dst, cm, kg, ds,
+                       nb, dbits, offsets, perm);
   GENPK_LAUNCH_CHECK();
 }
 
-template <int NM, int NI>
-TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, const float *masses, bool every_tile) {
-  constexpr int RW = Src<NM, NI>::RW;
+template <int NM>
+TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, const float *masses, bool with_perm,
+                    bool every_tile) {
   hipStream_t s = p.stream;
   const TileGeom &tg = p.tg;
 
   float *A = p.recA.as<float>();
   float *B = p.recB.as<float>();
   const int64_t g1 = ceil_div(n, SORT_CHUNK);
-  Src<NM, NI> raw{pos, masses, true};
+  Src<NM> raw{pos, masses, true};
+  uint32_t *perm1 = with_perm ? p.permA.as<uint32_t>() : nullptr;
+  uint32_t *perm2 = (with_perm && tg.passes == 2) ? p.permB.as<uint32_t>() : nullptr;
   const float *sorted = nullptr;
   uint32_t *tile_start = nullptr;
   int64_t ks = 0;
@@ -391,13 +395,13 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     ChunkMap cm{n, 0, nullptr, nullptr, g1, 0};
     {
       Stage st(p, "sort_upsweep1");
-      hipLaunchKernelGGL((k_upsweep<NM, NI>), dim3((unsigned)g1), dim3(SORT_THREADS), nb * 4, s, raw, cm, kg, dsel, nb, t1);
+      hipLaunchKernelGGL((k_upsweep<NM>), dim3((unsigned)g1), dim3(SORT_THREADS), nb * 4, s, raw, cm, kg, dsel, nb, t1);
       GENPK_LAUNCH_CHECK();
     }
     exclusive_scan_u32(t1, t1, (int64_t)nb * g1, p.scan_tmp, s);
     {
       Stage st(p, "sort_downsweep1");
-      launch_downsweep<NM, NI>(g1, s, raw, A, cm, kg, dsel, nb, tg.hi_bits, t1);
+      launch_downsweep<NM>(g1, s, raw, A, cm, kg, dsel, nb, tg.hi_bits, t1, perm1);
     }
     tile_start = static_cast<uint32_t *>(p.tile_start.ensure((size_t)(ks + 1) * 4));
     hipLaunchKernelGGL(k_tile_start_1pass, dim3((unsigned)ceil_div(ks + 1, 256)), dim3(256), 0, s, t1, g1, (int)ks,
@@ -413,14 +417,14 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     ChunkMap cm1{n, 0, nullptr, nullptr, g1, 0};
     {
       Stage st(p, "sort_upsweep1");
-      hipLaunchKernelGGL((k_upsweep<NM, NI>), dim3((unsigned)g1), dim3(SORT_THREADS), nhi * 4, s, raw, cm1, kg, d1, nhi,
+      hipLaunchKernelGGL((k_upsweep<NM>), dim3((unsigned)g1), dim3(SORT_THREADS), nhi * 4, s, raw, cm1, kg, d1, nhi,
                          t1);
       GENPK_LAUNCH_CHECK();
     }
     exclusive_scan_u32(t1, t1, (int64_t)nhi * g1, p.scan_tmp, s);
     {
       Stage st(p, "sort_downsweep1");
-      launch_downsweep<NM, NI>(g1, s, raw, A, cm1, kg, d1, nhi, tg.hi_bits, t1);
+      launch_downsweep<NM>(g1, s, raw, A, cm1, kg, d1, nhi, tg.hi_bits, t1, perm1);
     }
     // segments of pass 2
     uint32_t *seg = static_cast<uint32_t *>(p.seginfo.ensure((size_t)(nhi + 1) * 3 * 4));
@@ -433,10 +437,10 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     uint32_t *t2 = static_cast<uint32_t *>(p.table2.ensure((size_t)g2 * nlo * 4));
     DigitSel d2{0, (uint32_t)(nlo - 1)};
     ChunkMap cm2{n, nhi, seg_start, chunk_start, 0, nlo};
-    Src<NM, NI> pa{A, nullptr, false};
+    Src<NM> pa{A, nullptr, false};
     {
       Stage st(p, "sort_upsweep2");
-      hipLaunchKernelGGL((k_upsweep<NM, NI>), dim3((unsigned)g2), dim3(SORT_THREADS), nlo * 4, s, pa, cm2, kg, d2, nlo,
+      hipLaunchKernelGGL((k_upsweep<NM>), dim3((unsigned)g2), dim3(SORT_THREADS), nlo * 4, s, pa, cm2, kg, d2, nlo,
                          t2);
       GENPK_LAUNCH_CHECK();
     }
@@ -449,7 +453,7 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     GENPK_LAUNCH_CHECK();
     {
       Stage st(p, "sort_downsweep2");
-      launch_downsweep<NM, NI>(g2, s, pa, B, cm2, kg, d2, nlo, tg.lo_bits, t2);
+      launch_downsweep<NM>(g2, s, pa, B, cm2, kg, d2, nlo, tg.lo_bits, t2, perm2);
     }
     sorted = B;
   }
@@ -471,9 +475,11 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
   const int64_t max_items = tg.K + ceil_div(n, DEPOSIT_ITEM_MAX);
   TileOrder o;
   o.rec = sorted;
-  o.rw = RW;
+  o.rw = Src<NM>::RW;
   o.has_mass = NM != 0;
-  o.has_index = NI != 0;
+  o.has_perm = with_perm;
+  o.perm1 = perm1;
+  o.perm2 = perm2;
   o.tile_start = tile_start;
   o.item_start = item_start;
   o.part_start = part_start;
@@ -496,18 +502,21 @@ KeyGeom key_geom(const Plan &p, double boxsize) {
   return g;
 }
 
-TileOrder sort_by_tile(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, const float *masses, bool with_index,
+TileOrder sort_by_tile(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, const float *masses, bool with_perm,
                        bool every_tile) {
   GENPK_REQUIRE(n > 0 && n <= ((int64_t)1 << 31), "sort_by_tile: particle count out of range");
-  const int rw = 3 + (masses ? 1 : 0) + (with_index ? 1 : 0);
-  const size_t rec_bytes = (size_t)n * rw * 4;
+  const int rw = 3 + (masses ? 1 : 0);
+  // with the permutations, both record buffers can also take the adjoint
+  // gather's output records (up to 4 floats)
+  const size_t rec_bytes = (size_t)n * (with_perm ? 4 : rw) * 4;
   p.recA.ensure(rec_bytes);
-  if (p.tg.passes == 2) p.recB.ensure(rec_bytes);
-  if (masses)
-    return with_index ? sort_impl<1, 1>(p, kg, n, pos, masses, every_tile)
-                      : sort_impl<1, 0>(p, kg, n, pos, masses, every_tile);
-  return with_index ? sort_impl<0, 1>(p, kg, n, pos, nullptr, every_tile)
-                    : sort_impl<0, 0>(p, kg, n, pos, nullptr, every_tile);
+  if (p.tg.passes == 2 || with_perm) p.recB.ensure(rec_bytes);
+  if (with_perm) {
+    p.permA.ensure((size_t)n * 4);
+    if (p.tg.passes == 2) p.permB.ensure((size_t)n * 4);
+  }
+  if (masses) return sort_impl<1>(p, kg, n, pos, masses, with_perm, every_tile);
+  return sort_impl<0>(p, kg, n, pos, nullptr, with_perm, every_tile);
 }
 
 }  // namespace genpk

@@ -67,11 +67,13 @@ __device__ __forceinline__ bool item_range(const uint32_t *__restrict__ item_sta
 
 // Sort the n particles of one call (n < 2^31) by tile into plan scratch and
 // build the work items (at most DEPOSIT_ITEM_MAX records each).  Records are
-// x, y, z, then the mass when masses != nullptr, then the particle's index in
-// [0, n) (as the bits of a float) when with_index.  every_tile: every tile
-// gets at least one item (the deposit then writes untouched tiles as
-// zeros); otherwise empty tiles get none.
-TileOrder sort_by_tile(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, const float *masses, bool with_index,
+// x, y, z, then the mass when masses != nullptr.  with_perm: also record
+// where every particle went (perm1: input -> pass-1 position; perm2, two
+// passes only: pass-1 -> final position), for taking results back to the
+// input order without scattered writes.  every_tile: every tile gets at
+// least one item (the deposit then writes untouched tiles as zeros);
+// otherwise empty tiles get none.
+TileOrder sort_by_tile(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, const float *masses, bool with_perm,
                        bool every_tile);
 
 }  // namespace genpk
