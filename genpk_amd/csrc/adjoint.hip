@@ -31,11 +31,11 @@ int col_block(int prec, int64_t d, int64_t hz);
 
 namespace {
 
-constexpr int ADJ_THREADS = 256;
+constexpr int ADJ_THREADS = 1024;  // as the transform's column passes
 constexpr int GATHER_THREADS = 512;
 template <typename T> struct AdjCfg;
-template <> struct AdjCfg<float> { static constexpr int MAXE = 32; };
-template <> struct AdjCfg<double> { static constexpr int MAXE = 16; };
+template <> struct AdjCfg<float> { static constexpr int MAXE = 8; };
+template <> struct AdjCfg<double> { static constexpr int MAXE = 4; };
 
 __device__ __forceinline__ int64_t kv(int64_t n, int64_t d) { return n <= d / 2 ? n : n - d; }
 
