@@ -30,11 +30,14 @@ int col_block(int prec, int64_t d, int64_t hz);
 
 namespace {
 
-constexpr int BIN_THREADS = 256;
+// 16 waves per workgroup with 8 staged values per thread (MAXE): few enough
+// registers (~80) for 16 resident waves per CU, twice the waves of 256 threads
+// x 32 values (x pass + bin 4.0 -> 3.4 ms at 1024^3).
+constexpr int BIN_THREADS = 1024;
 constexpr int LDS_LIMIT = 160 * 1024;
 template <typename T> struct BinCfg;
-template <> struct BinCfg<float> { static constexpr int ELEMS = 8192; static constexpr int MAXE = 32; };
-template <> struct BinCfg<double> { static constexpr int ELEMS = 4096; static constexpr int MAXE = 16; };
+template <> struct BinCfg<float> { static constexpr int ELEMS = 8192; static constexpr int MAXE = 8; };
+template <> struct BinCfg<double> { static constexpr int ELEMS = 4096; static constexpr int MAXE = 4; };
 
 __device__ __forceinline__ int64_t kval(int64_t n, int64_t d) { return n <= d / 2 ? n : n - d; }
 
@@ -147,7 +150,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
   const int64_t estride = d * hz;
   const int64_t nblk = d * nkb;
   // bin-phase mapping: thread -> column b, contiguous run of e
-  const int tpc = BIN_THREADS / B;               // threads per column (B divides 256)
+  const int tpc = BIN_THREADS / B;               // threads per column (B divides BIN_THREADS)
   const int my_b = threadIdx.x % B;
   const int per = (n + tpc - 1) / tpc;
   const int e0 = (threadIdx.x / B) * per;
@@ -411,6 +414,24 @@ static int fused_block(int prec, int64_t d, int64_t hz, int nf, int ns, int nrbi
 }
 
 template <typename T, int NF>
+static size_t fused_lds(const Plan &a, int B, int nrbins) {
+  using C = typename Cx<T>::C;
+  constexpr int NS = NF == 1 ? 1 : 3;
+  return ((size_t)NF * B + 1) * a.d * sizeof(C) + (size_t)((NS * nrbins + 1) & ~1) * 8;
+}
+
+// Persistent grid of the fused kernel: as many workgroups as stay resident
+// (occupancy of this instance x CUs), at most one per column block.
+template <typename T, int NF>
+static int fused_grid(const Plan &a, int B, int nrbins) {
+  int per_cu = 0;
+  GENPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_xpass_bin<T, NF>),
+                                                          BIN_THREADS, fused_lds<T, NF>(a, B, nrbins)));
+  const int64_t nblk = a.d * ceil_div(a.hz, B);
+  return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)std::max(per_cu, 1) * num_cus(), nblk));
+}
+
+template <typename T, int NF>
 static void launch_fused(Plan &a, Plan *b, const BinTables &bt, int B, bool keep, double *part, int G) {
   using C = typename Cx<T>::C;
   constexpr int NS = NF == 1 ? 1 : 3;
@@ -418,7 +439,7 @@ static void launch_fused(Plan &a, Plan *b, const BinTables &bt, int B, bool keep
   const RadixPlan rp = make_radix_plan((int)a.d);
   const C *tw = (a.prec == 4) ? (const C *)a.fft.tw_d_f32.ptr : (const C *)a.fft.tw_d_f64.ptr;
   BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), bt.thr.as<int64_t>(), a.wtab.as<float>()};
-  const size_t lds = ((size_t)NF * B + 1) * a.d * sizeof(C) + (size_t)((NS * bt.nrbins + 1) & ~1) * 8;
+  const size_t lds = fused_lds<T, NF>(a, B, bt.nrbins);
   Stage st(a, "xpass_bin");
   hipLaunchKernelGGL((k_xpass_bin<T, NF>), dim3(G), dim3(BIN_THREADS), lds, a.stream, a.field.as<C>(),
                      b ? b->field.as<C>() : (C *)nullptr, g, B, nkb, rp, tw, keep ? 1 : 0, part);
@@ -444,13 +465,18 @@ void bin_spectra(Plan &a, Plan *b, int nrbins, bool fused_xpass, double *pow_aa,
   }
   const BinTables &bt = bin_tables(a, nrbins);
   const int nf = b ? 2 : 1, ns = b ? 3 : 1;
-  const int G = std::max(1, std::min<int>(2 * num_cus(), (int)std::min<int64_t>(a.d * a.d, 1 << 20)));
-  double *part = static_cast<double *>(a.bin_part.ensure((size_t)G * ns * nrbins * 8));
+  int G = std::max(1, std::min<int>(2 * num_cus(), (int)std::min<int64_t>(a.d * a.d, 1 << 20)));
+  double *part = nullptr;
   if (fused_xpass) {
     GENPK_REQUIRE(a.fstate == FieldState::YDone && (!b || b->fstate == FieldState::YDone),
                   "fused binning needs fields after the z and y passes");
     const int B = fused_block(a.prec, a.d, a.hz, nf, ns, nrbins);
     GENPK_REQUIRE(B > 0, "fused x-pass does not fit in LDS for this size");
+    if (a.prec == 4)
+      G = b ? fused_grid<float, 2>(a, B, nrbins) : fused_grid<float, 1>(a, B, nrbins);
+    else
+      G = b ? fused_grid<double, 2>(a, B, nrbins) : fused_grid<double, 1>(a, B, nrbins);
+    part = static_cast<double *>(a.bin_part.ensure((size_t)G * ns * nrbins * 8));
     if (a.prec == 4) {
       if (b) launch_fused<float, 2>(a, b, bt, B, false, part, G); else launch_fused<float, 1>(a, b, bt, B, false, part, G);
     } else {
@@ -462,6 +488,7 @@ void bin_spectra(Plan &a, Plan *b, int nrbins, bool fused_xpass, double *pow_aa,
   } else {
     GENPK_REQUIRE(a.fstate == FieldState::Spectrum && (!b || b->fstate == FieldState::Spectrum),
                   "binning needs transformed fields");
+    part = static_cast<double *>(a.bin_part.ensure((size_t)G * ns * nrbins * 8));
     if (a.prec == 4) {
       if (b) launch_stored<float, 2>(a, b, bt, part, G); else launch_stored<float, 1>(a, b, bt, part, G);
     } else {

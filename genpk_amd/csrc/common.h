@@ -59,12 +59,12 @@ int guarded(F &&f) {
 // Fixed-point scale of the LDS deposit accumulators: CIC weights in [0, 1]
 // are accumulated as unsigned 64-bit integers with 36 fractional bits
 // (resolution 1.5e-11 of a particle's mass).  A work item holds at most
-// DEPOSIT_ITEM_MAX = 2^16 particles, so a cell sum stays at or below 2^52
+// DEPOSIT_ITEM_MAX = 2^17 particles, so a cell sum stays at or below 2^53
 // and converts to double exactly.
 constexpr int FIXED_FRAC_BITS = 36;
 constexpr double FIXED_ONE = 68719476736.0;            // 2^36
 constexpr double FIXED_INV = 1.0 / 68719476736.0;      // 2^-36
-constexpr int DEPOSIT_ITEM_MAX = 65536;
+constexpr int DEPOSIT_ITEM_MAX = 131072;
 
 template <typename T> struct cplx_of;
 template <> struct cplx_of<float> { using type = float2; };
