@@ -154,38 +154,49 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
   const int my_b = threadIdx.x % B;
   const int per = (n + tpc - 1) / tpc;
   const int e0 = (threadIdx.x / B) * per;
-  for (int64_t v = blockIdx.x; v < nblk; v += gridDim.x) {
+  // register-staged loads of column block v: all its global reads in flight together
+  constexpr int MAXE = BinCfg<T>::MAXE;
+  const int tot = n * B;
+  C va[MAXE], vb[NF == 2 ? MAXE : 1];
+  auto load = [&](int64_t v) {
     // a grid that is a multiple of the XCD count keeps v on this workgroup's XCD
+    const int64_t blk = xcd_block((uint32_t)v, (uint32_t)nblk);
+    const int64_t y = blk / nkb;
+    const int64_t kz0 = (blk % nkb) * B;
+    const int nb = (int)min((int64_t)B, hz - kz0);
+    const int64_t off = y * hz + kz0;
+#pragma unroll
+    for (int q = 0; q < MAXE; q++) {
+      const int i = threadIdx.x + q * BIN_THREADS;
+      const int e = i >> lgb, b = i & (B - 1);
+      const bool ok = i < tot && b < nb;
+      va[q] = ok ? fa[off + e * estride + b] : C{0, 0};
+      if (NF == 2) vb[NF == 2 ? q : 0] = ok ? fb[off + e * estride + b] : C{0, 0};
+    }
+  };
+  // one field: the next block's loads overlap this block's FFT and binning
+  // (two fields would hold twice the registers across the FFT and spill)
+  constexpr bool PREFETCH = NF == 1;
+  if (PREFETCH && (int64_t)blockIdx.x < nblk) load(blockIdx.x);
+  for (int64_t v = blockIdx.x; v < nblk; v += gridDim.x) {
     const int64_t blk = xcd_block((uint32_t)v, (uint32_t)nblk);
     const int64_t y = blk / nkb;
     const int kb = (int)(blk % nkb);
     const int64_t kz0 = (int64_t)kb * B;
     const int nb = (int)min((int64_t)B, hz - kz0);
     const int64_t off = y * hz + kz0;
-    {
-      // register-staged loads: all global reads of the block are in flight together
-      constexpr int MAXE = BinCfg<T>::MAXE;
-      const int tot = n * B;
-      C va[MAXE], vb[NF == 2 ? MAXE : 1];
+    if (!PREFETCH) load(v);
+    __syncthreads();  // the previous block's binning is done with buf
 #pragma unroll
-      for (int q = 0; q < MAXE; q++) {
-        const int i = threadIdx.x + q * BIN_THREADS;
-        const int e = i >> lgb, b = i & (B - 1);
-        const bool ok = i < tot && b < nb;
-        va[q] = ok ? fa[off + e * estride + b] : C{0, 0};
-        if (NF == 2) vb[NF == 2 ? q : 0] = ok ? fb[off + e * estride + b] : C{0, 0};
-      }
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < MAXE; q++) {
-        const int i = threadIdx.x + q * BIN_THREADS;
-        if (i < tot) {
-          buf[i] = va[q];
-          if (NF == 2) buf[tot + i] = vb[NF == 2 ? q : 0];
-        }
+    for (int q = 0; q < MAXE; q++) {
+      const int i = threadIdx.x + q * BIN_THREADS;
+      if (i < tot) {
+        buf[i] = va[q];
+        if (NF == 2) buf[tot + i] = vb[NF == 2 ? q : 0];
       }
     }
     __syncthreads();
+    if (PREFETCH && v + gridDim.x < nblk) load(v + gridDim.x);
     lds_fft<-1, true, BinCfg<T>::MAXE>(buf, rp, B, lgb, B, 1, stw);
     if (NF == 2) lds_fft<-1, true, BinCfg<T>::MAXE>(buf + n * B, rp, B, lgb, B, 1, stw);
     if (keep) {
@@ -214,10 +225,10 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
         const float gw = (g.wtab[e] * wy) * wz;   // the reference's ((wx*wy)*wz) in float
         double dc = (double)gw * (double)gw;
         dc *= dc;
-        const C va = buf[e * B + my_b];
-        const C vb = NF == 2 ? buf[n * B + e * B + my_b] : va;
+        const C ma = buf[e * B + my_b];
+        const C mb = NF == 2 ? buf[n * B + e * B + my_b] : ma;
         double c[3];
-        mode_terms<C, NF>(va, vb, h, dc, c);
+        mode_terms<C, NF>(ma, mb, h, dc, c);
         run_add<NS>(r, b, c, bins, g.nrbins);
       }
       run_flush<NS>(r, bins, g.nrbins);

@@ -1,22 +1,22 @@
 // Partition of particles by deposit tile on gfx950: an MSD radix partition
 // of one or two passes over the tile key (upsweep histogram in LDS ->
-// offset scan -> downsweep that ranks records with wavefront ballots,
-// stages them in LDS by digit and writes contiguous runs), then the work
-// items of the tile kernels.  Global atomics are avoided: on this chip they
-// run at ~27 Gop/s.  The partition is stable, so the records of one tile
-// keep their input order.
+// offset scan -> downsweep that ranks records with LDS atomics, stages them
+// in LDS by digit and writes contiguous runs), then the work items of the
+// tile kernels.  Global atomics are avoided: on this chip they run at
+// ~27 Gop/s.  The order of the records within one tile is not specified
+// (the tile kernels do not depend on it).
 #include "tilesort.h"
 
 namespace genpk {
 
 namespace {
 
-// 8 waves per workgroup.  The downsweep's records per lane per sub-chunk (PL)
-// is the largest of 8, 4, 2 whose LDS staging leaves room for two workgroups
-// (16 waves) per CU: at 1024^3 this measured faster than 4 waves x 16 records
-// (8.3 -> 7.2 ms per downsweep) or 16 waves x 4 records, and one workgroup
-// per CU (16-byte records staged 4096 at a time) doubles the time.
-constexpr int SORT_THREADS = 512;
+// 16 waves per workgroup, one workgroup per CU in the downsweep: a
+// sub-chunk of 1024 * PL records is staged in LDS at a time and written out
+// as one run per digit, and longer runs write faster (at 2^30 records and
+// 256 digits, 8192-record sub-chunks take 6.9 ms per downsweep against 7.2
+// ms for two 512-thread workgroups per CU staging 4096 records each).
+constexpr int SORT_THREADS = 1024;
 constexpr int SORT_WAVES = SORT_THREADS / 64;
 constexpr int SORT_CHUNK = 65536;                                   // records per workgroup
 
@@ -89,12 +89,46 @@ struct ChunkMap {
 struct DigitSel {
   int shift;
   uint32_t mask;
-  __device__ __forceinline__ uint32_t of(uint32_t key) const { return (key >> shift) & mask; }
+  int axes;  // bit a set: the digit depends on the tile index along axis a (x, y, z)
+  // digit of a record; cells along axes the digit does not depend on are not computed
+  __device__ __forceinline__ uint32_t of(const float *r, const KeyGeom &g) const {
+    double fr;
+    const int cx = (axes & 1) ? cell_of(r[0], g.units, g.d, fr) : 0;
+    const int cy = (axes & 2) ? cell_of(r[1], g.units, g.d, fr) : 0;
+    const int cz = (axes & 4) ? cell_of(r[2], g.units, g.d, fr) : 0;
+    const uint32_t key = (uint32_t)((((cx >> g.sx) * g.nty + (cy >> g.sy)) * g.ntz) + (cz >> g.sz));
+    return (key >> shift) & mask;
+  }
 };
+
+int log2_floor(int v) {
+  int l = 0;
+  while ((1 << (l + 1)) <= v) l++;
+  return l;
+}
+
+// Digit of key bits [shift, shift + width).  The key is (tx * nty + ty) * ntz
+// + tz; when nty and ntz are powers of two its bits are tz, then ty, then tx,
+// and a digit needs only the axes whose bits it covers.
+DigitSel digit_sel(const KeyGeom &g, int shift, int width) {
+  DigitSel s{shift, (uint32_t)((1u << width) - 1), 7};
+  if ((g.nty & (g.nty - 1)) != 0 || (g.ntz & (g.ntz - 1)) != 0) return s;
+  const int bz = log2_floor(g.ntz), by = log2_floor(g.nty);
+  const int lo[3] = {bz + by, bz, 0}, hi[3] = {32, bz + by, bz};
+  s.axes = 0;
+  for (int a = 0; a < 3; a++)
+    if (lo[a] < hi[a] && lo[a] < shift + width && shift < hi[a]) s.axes |= 1 << a;
+  return s;
+}
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const int lane = threadIdx.x & 63;
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// number of set bits of m below this lane
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 // counts[tb + dg * ts] = number of records of chunk g with digit dg
@@ -118,50 +152,133 @@ __global__ void __launch_bounds__(SORT_THREADS) k_upsweep(Src<NM> src, ChunkMap 
     for (int u = 0; u < UNR; u++) {
       const int64_t i = base + u * SORT_THREADS + threadIdx.x;
       const bool valid = i < e;
-      const uint32_t dg = valid ? ds.of(tile_key(r[u], kg)) : 0u;
+      const uint32_t dg = valid ? ds.of(r[u], kg) : 0u;
       // a wavefront whose records share one digit adds them with one atomic
       const uint64_t vmask = __ballot(valid);
-      const uint32_t d0 = __shfl(dg, (int)__builtin_ctzll(vmask | (1ull << 63)), 64);
+      const uint32_t d0 = __builtin_amdgcn_readlane(dg, (int)__builtin_ctzll(vmask | (1ull << 63)));
       if (__ballot(valid && dg == d0) == vmask) {
         if (valid && (vmask & lanemask_lt()) == 0) atomicAdd(&hist[d0], (uint32_t)__popcll(vmask));
       } else if (valid) {
         atomicAdd(&hist[dg], 1u);
       }
-    }  }
+    }
+  }
   __syncthreads();
   for (int i = threadIdx.x; i < nb; i += SORT_THREADS)
     counts[tb + (int64_t)i * ts] = hist[i];
 }
 
-// Stable partition of each chunk by digit into dst at offsets[tb + dg * ts].
-// perm (optional): perm[i] = output position of input record i.
-// Per sub-chunk of SORT_THREADS * PL records (each wave owns 64 * PL consecutive ones):
-// wavefront ballots give every record its rank among equal digits, LDS
-// prefix sums turn ranks into positions of a digit-sorted staging copy, and
-// the staging copy is written out as contiguous runs.  The next sub-chunk's
-// records are loaded while the current one is written out.
+// Rank of each of a lane's PL records among the wave's records of its digit
+// in this sub-chunk, from LDS atomic adds that return the old count on the
+// wave's count row (a wave whose records share one digit adds them with one
+// atomic).  Equal digits are ordered by the LDS unit's processing order of
+// one instruction's lanes, so the partition is not guaranteed to be stable;
+// per-wave count rows keep the result independent of wave scheduling.
 template <int NM, int PL>
-__global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<NM> src, float *__restrict__ dst, ChunkMap cm,
-                                                            KeyGeom kg, DigitSel ds, int nb, int dbits,
-                                                            const uint32_t *__restrict__ offsets,
-                                                            uint32_t *__restrict__ perm) {
+__device__ __forceinline__ void rank_records(const float (&rec)[PL][Src<NM>::RW], int wbase, int nsub,
+                                             const KeyGeom &kg, const DigitSel &ds, uint32_t *wrow,
+                                             uint32_t (&dig)[PL], uint32_t (&rnk)[PL]) {
+  const int lane = threadIdx.x & 63;
+  uint64_t gpeers[PL];
+  int glead[PL];
+#pragma unroll
+  for (int it = 0; it < PL; it++) {
+    const bool valid = wbase + it * 64 < nsub;
+    const uint32_t dg = valid ? ds.of(rec[it], kg) : 0u;
+    const uint64_t vmask = __ballot(valid);
+    const int lead = (int)__builtin_ctzll(vmask | (1ull << 63));
+    const uint32_t d0 = __builtin_amdgcn_readlane(dg, lead);
+    uint32_t r = 0;
+    if (__ballot(valid && dg == d0) == vmask) {
+      if (valid && lane == lead) r = atomicAdd(&wrow[d0], (uint32_t)__popcll(vmask));
+      gpeers[it] = vmask;
+      glead[it] = lead;
+    } else {
+      if (valid) r = atomicAdd(&wrow[dg], 1u);
+      gpeers[it] = 0;
+      glead[it] = -1;
+    }
+    dig[it] = dg;
+    rnk[it] = r;
+  }
+  // a one-digit wave's records follow its leader's old count
+#pragma unroll
+  for (int it = 0; it < PL; it++)
+    if (glead[it] >= 0) rnk[it] = __builtin_amdgcn_readlane(rnk[it], glead[it]) + mbcnt(gpeers[it]);
+}
+
+// Per-row counts of digit dg -> per-row exclusive offsets in place; returns the total.
+__device__ __forceinline__ uint32_t wave_offsets(uint32_t *whist, int nb, int hw, int dg) {
+  uint32_t tot = 0;
+  for (int w = 0; w < hw; w++) {
+    const uint32_t t = whist[w * nb + dg];
+    whist[w * nb + dg] = tot;
+    tot += t;
+  }
+  return tot;
+}
+
+// One round of a block-wide exclusive scan (one value per thread), added to
+// and advancing the running total 'carry'.  Contains a barrier; wtot rows
+// alternate between rounds.
+__device__ __forceinline__ uint32_t block_scan_round(uint32_t v, uint32_t (*wtot)[SORT_WAVES], int rd,
+                                                     uint32_t &carry) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += t;
+  }
+  if (lane == 63) wtot[rd & 1][wave] = incl;
+  __syncthreads();
+  uint32_t run = carry + incl - v, all = 0;
+#pragma unroll
+  for (int w = 0; w < SORT_WAVES; w++) {
+    const uint32_t t = wtot[rd & 1][w];
+    if (w < wave) run += t;
+    all += t;
+  }
+  carry += all;
+  return run;
+}
+
+// Partition of each chunk by digit into dst at offsets[tb + dg * ts].  perm (optional): perm[i] = output position of
+// input record i.  Per sub-chunk of SORT_THREADS * PL records (each wave
+// owns 64 * PL consecutive ones): ranks (rank_records), then one thread per
+// digit turns the per-wave counts into offsets and a block scan gives the
+// digit's start in a digit-sorted staging copy and dbase = (global output
+// offset - staging start) mod 2^32, records go to the staging copy, and the
+// staging copy is written out as contiguous runs (record p goes to
+// dbase[its digit] + p) while the next sub-chunk's records load.  With DB
+// the per-wave counts are double-buffered: this sub-chunk's buffer is
+// cleared during the write-out and the next sub-chunk ranks into the other
+// one, so no barrier separates them.  hw count rows (a divisor of the wave
+// count) serve the waves, wave w on row w % hw: with fewer rows than waves
+// (wide digits) the order of equal digits also depends on wave scheduling.
+template <int NM, int PL, bool DB>
+__global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, float *__restrict__ dst, ChunkMap cm,
+                                                               KeyGeom kg, DigitSel ds, int nb, int hw,
+                                                               const uint32_t *__restrict__ offsets,
+                                                               uint32_t *__restrict__ perm) {
   constexpr int RW = Src<NM>::RW;
-  constexpr int SUBN = SORT_THREADS * PL;  // records per sub-chunk
+  constexpr int SUBN = SORT_THREADS * PL;
+  constexpr int NBUF = DB ? 2 : 1;
   extern __shared__ uint32_t sm[];
-  uint32_t *whist = sm;                         // [SORT_WAVES][nb]
-  uint32_t *lstart = whist + SORT_WAVES * nb;   // [nb]
-  uint32_t *gbase = lstart + nb;                // [nb]
-  uint32_t *subcnt = gbase + nb;                // [nb]
-  uint16_t *sdig = reinterpret_cast<uint16_t *>(subcnt + nb);          // [SUBN]
-  float *stage = reinterpret_cast<float *>(sdig + SUBN);           // [SUBN][RW]
-  __shared__ uint32_t wtot[SORT_WAVES];
+  uint32_t *whist_all = sm;                                  // [NBUF][hw][nb]
+  uint32_t *lstart = whist_all + NBUF * hw * nb;             // [nb]
+  uint32_t *gbase = lstart + nb;                             // [nb]
+  uint32_t *dbase = gbase + nb;                              // [nb]
+  uint16_t *sdig = reinterpret_cast<uint16_t *>(dbase + nb);  // [SUBN]
+  float *stage = reinterpret_cast<float *>(sdig + SUBN);      // [SUBN][RW]
+  __shared__ uint32_t wtot[2][SORT_WAVES];
 
   int64_t cb, ce, tb, ts;
   if (!cm.range(blockIdx.x, cb, ce, tb, ts)) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int i = threadIdx.x; i < nb; i += SORT_THREADS)
-    gbase[i] = offsets[tb + (int64_t)i * ts];
-  for (int i = threadIdx.x; i < SORT_WAVES * nb; i += SORT_THREADS) whist[i] = 0;
+  for (int i = threadIdx.x; i < nb; i += SORT_THREADS) gbase[i] = offsets[tb + (int64_t)i * ts];
+  for (int i = threadIdx.x; i < NBUF * hw * nb; i += SORT_THREADS) whist_all[i] = 0;
+  const int row = (wave % hw) * nb;
 
   float rec[PL][RW];
   const int wbase = wave * (PL * 64) + lane;
@@ -172,71 +289,26 @@ __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<NM> src, float *
   }
   __syncthreads();
 
-  for (int64_t s0 = cb; s0 < ce; s0 += SUBN) {
+  int sub = 0;
+  for (int64_t s0 = cb; s0 < ce; s0 += SUBN, sub++) {
     const int nsub = (int)min((int64_t)SUBN, ce - s0);
-    uint32_t dig[PL];
-    uint32_t rnk[PL];
-#pragma unroll
-    for (int it = 0; it < PL; it++) {
-      const int loc = wbase + it * 64;
-      const bool valid = loc < nsub;
-      const uint32_t dg = valid ? ds.of(tile_key(rec[it], kg)) : 0u;
-      uint64_t peers = __ballot(valid);
-      // wavefronts whose records share one digit skip the per-bit match
-      const uint32_t d0 = __shfl(dg, (int)__builtin_ctzll(peers | (1ull << 63)), 64);
-      if (__ballot(valid && dg == d0) != peers) {
-        for (int bit = 0; bit < dbits; bit++) {
-          const bool set = (dg >> bit) & 1u;
-          const uint64_t bb = __ballot(set);
-          peers &= set ? bb : ~bb;
-        }
-      }
-      dig[it] = dg;
-      rnk[it] = 0;
-      if (valid) {
-        const uint32_t before = whist[wave * nb + dg];
-        rnk[it] = before + (uint32_t)__popcll(peers & lanemask_lt());
-        // every lane has read the running count before the group leader updates it
-        __builtin_amdgcn_wave_barrier();
-        if ((peers & lanemask_lt()) == 0) whist[wave * nb + dg] = before + (uint32_t)__popcll(peers);
-      }
-      __builtin_amdgcn_wave_barrier();
-    }
+    uint32_t *whist = whist_all + (DB ? (sub & 1) * hw * nb : 0);
+    uint32_t dig[PL], rnk[PL];
+    rank_records<NM, PL>(rec, wbase, nsub, kg, ds, whist + row, dig, rnk);
     __syncthreads();
-    // per-digit exclusive prefix over waves and sub-chunk totals
-    for (int dgi = threadIdx.x; dgi < nb; dgi += SORT_THREADS) {
-      uint32_t tot = 0;
-#pragma unroll
-      for (int w = 0; w < SORT_WAVES; w++) {
-        const uint32_t t = whist[w * nb + dgi];
-        whist[w * nb + dgi] = tot;
-        tot += t;
-      }
-      subcnt[dgi] = tot;
-    }
-    __syncthreads();
-    // exclusive scan of subcnt -> lstart (nb <= 2048)
     {
-      const int per = (nb + SORT_THREADS - 1) / SORT_THREADS;
-      const int first = threadIdx.x * per;
-      uint32_t tsum = 0;
-      for (int i = 0; i < per; i++)
-        if (first + i < nb) tsum += subcnt[first + i];
-      uint32_t incl = tsum;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t t = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += t;
-      }
-      if (lane == 63) wtot[wave] = incl;
-      __syncthreads();
-      uint32_t run = incl - tsum;
-      for (int w = 0; w < wave; w++) run += wtot[w];
-      for (int i = 0; i < per; i++)
-        if (first + i < nb) {
-          lstart[first + i] = run;
-          run += subcnt[first + i];
+      uint32_t total = 0;
+      for (int r0 = 0, rd = 0; r0 < nb; r0 += SORT_THREADS, rd++) {
+        const int dg = r0 + threadIdx.x;
+        const uint32_t tot = dg < nb ? wave_offsets(whist, nb, hw, dg) : 0u;
+        const uint32_t run = block_scan_round(tot, wtot, rd, total);
+        if (dg < nb) {
+          const uint32_t g = gbase[dg];
+          lstart[dg] = run;
+          dbase[dg] = g - run;
+          gbase[dg] = g + tot;
         }
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -244,16 +316,14 @@ __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<NM> src, float *
       const int loc = wbase + it * 64;
       if (loc < nsub) {
         const uint32_t dg = dig[it];
-        const uint32_t within = whist[wave * nb + dg] + rnk[it];  // place in the digit's run of this sub-chunk
-        const uint32_t p = lstart[dg] + within;
+        const uint32_t p = lstart[dg] + whist[row + dg] + rnk[it];
 #pragma unroll
         for (int k = 0; k < RW; k++) stage[p * RW + k] = rec[it][k];
         sdig[p] = (uint16_t)dg;
-        if (perm) perm[s0 + loc] = gbase[dg] + within;  // where this input record goes (coalesced)
+        if (perm) perm[s0 + loc] = dbase[dg] + p;  // where this input record goes (coalesced)
       }
     }
     __syncthreads();
-    // prefetch the next sub-chunk while this one is written out
     {
       const int64_t n0 = s0 + SUBN;
 #pragma unroll
@@ -262,16 +332,22 @@ __global__ void __launch_bounds__(SORT_THREADS) k_downsweep(Src<NM> src, float *
         if (i < ce) src.load(i, rec[it]);
       }
     }
-    for (int p = threadIdx.x; p < nsub; p += SORT_THREADS) {
-      const uint32_t dg = sdig[p];
-      const int64_t dest = (int64_t)gbase[dg] + (p - (int64_t)lstart[dg]);
+    auto put = [&](int p) {
+      const uint32_t dest = dbase[sdig[p]] + (uint32_t)p;
+      float v[RW];
 #pragma unroll
-      for (int k = 0; k < RW; k++) dst[dest * RW + k] = stage[p * RW + k];
+      for (int k = 0; k < RW; k++) v[k] = stage[p * RW + k];
+#pragma unroll
+      for (int k = 0; k < RW; k++) dst[(int64_t)dest * RW + k] = v[k];
+    };
+    if (nsub == SUBN) {
+#pragma unroll
+      for (int it = 0; it < PL; it++) put(threadIdx.x + it * SORT_THREADS);
+    } else {
+      for (int p = threadIdx.x; p < nsub; p += SORT_THREADS) put(p);
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < nb; i += SORT_THREADS) gbase[i] += subcnt[i];
-    for (int i = threadIdx.x; i < SORT_WAVES * nb; i += SORT_THREADS) whist[i] = 0;
-    __syncthreads();
+    for (int i = threadIdx.x; i < hw * nb; i += SORT_THREADS) whist[i] = 0;
+    if (!DB) __syncthreads();
   }
 }
 
@@ -342,32 +418,49 @@ __global__ void k_heavy_list(const uint32_t *__restrict__ pcnt, int64_t ks, uint
   if (pcnt[k] > 1) heavy[atomicAdd(nheavy, 1u)] = (uint32_t)k;
 }
 
-size_t downsweep_lds(int nb, int rw, int pl) {
-  const size_t sub = (size_t)SORT_THREADS * pl;
-  return (size_t)(SORT_WAVES * nb + 3 * nb) * 4 + sub * 2 + sub * rw * 4;
+struct DsShape {
+  int pl;
+  bool db;
+  int hw;
+};
+
+size_t downsweep_lds(int nb, int rw, const DsShape &h) {
+  const size_t sub = (size_t)SORT_THREADS * h.pl;
+  return (size_t)((h.db ? 2 : 1) * h.hw * nb + 3 * nb) * 4 + sub * 2 + sub * rw * 4;
 }
 
-int downsweep_pl(int nb, int rw) {
-  for (int pl : {8, 4})
-    if (downsweep_lds(nb, rw, pl) <= 80 * 1024) return pl;
-  return 2;
+// The first shape whose LDS fits one workgroup per CU: a count row per wave
+// when possible, then the most records per lane, then double-buffered counts.
+DsShape downsweep_shape(int nb, int rw) {
+  const size_t cap = 160 * 1024 - 256;
+  for (int hw : {SORT_WAVES, SORT_WAVES / 4})
+    for (int pl : {8, 4, 2})
+      for (bool db : {true, false}) {
+        const DsShape h{pl, db, hw};
+        if (downsweep_lds(nb, rw, h) <= cap) return h;
+      }
+  return DsShape{1, false, 1};
 }
 
 template <int NM>
 void launch_downsweep(int64_t grid, hipStream_t s, Src<NM> src, float *dst, const ChunkMap &cm, const KeyGeom &kg,
-                      const DigitSel &ds, int nb, int dbits, const uint32_t *offsets, uint32_t *perm) {
-  constexpr int RW = Src<NM>::RW;
-  const int pl = downsweep_pl(nb, RW);
-  const size_t lds = downsweep_lds(nb, RW, pl);
-  if (pl == 8)
-    hipLaunchKernelGGL((k_downsweep<NM, 8>), dim3((unsigned)grid), dim3(SORT_THREADS), lds, s, src, dst, cm, kg, ds,
-                       nb, dbits, offsets, perm);
-  else if (pl == 4)
-    hipLaunchKernelGGL((k_downsweep<NM, 4>), dim3((unsigned)grid), dim3(SORT_THREADS), lds, s, src, dst, cm, kg, ds,
-                       nb, dbits, offsets, perm);
+                      const DigitSel &ds, int nb, const uint32_t *offsets, uint32_t *perm) {
+  GENPK_REQUIRE(nb <= 2 * SORT_THREADS, "sort digit too wide");
+  const DsShape h = downsweep_shape(nb, Src<NM>::RW);
+  const size_t lds = downsweep_lds(nb, Src<NM>::RW, h);
+  const dim3 g((unsigned)grid), b(SORT_THREADS);
+  if (h.pl == 8 && h.db)
+    hipLaunchKernelGGL((k_downsweep<NM, 8, true>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm);
+  else if (h.pl == 8)
+    hipLaunchKernelGGL((k_downsweep<NM, 8, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm);
+  else if (h.pl == 4 && h.db)
+    hipLaunchKernelGGL((k_downsweep<NM, 4, true>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm);
+  else if (h.pl == 4)
+    hipLaunchKernelGGL((k_downsweep<NM, 4, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm);
+  else if (h.pl == 2)
+    hipLaunchKernelGGL((k_downsweep<NM, 2, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm);
   else
-    hipLaunchKernelGGL((k_downsweep<NM, 2>), dim3((unsigned)grid), dim3(SORT_THREADS), lds, s, src, dst, cm, kg, ds,
-                       nb, dbits, offsets, perm);
+    hipLaunchKernelGGL((k_downsweep<NM, 1, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm);
   GENPK_LAUNCH_CHECK();
 }
 
@@ -391,7 +484,7 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     const int nb = 1 << tg.hi_bits;
     ks = nb;
     uint32_t *t1 = static_cast<uint32_t *>(p.table1.ensure((size_t)nb * g1 * 4));
-    DigitSel dsel{0, (uint32_t)(nb - 1)};
+    const DigitSel dsel = digit_sel(kg, 0, tg.hi_bits);
     ChunkMap cm{n, 0, nullptr, nullptr, g1, 0};
     {
       Stage st(p, "sort_upsweep1");
@@ -401,7 +494,7 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     exclusive_scan_u32(t1, t1, (int64_t)nb * g1, p.scan_tmp, s);
     {
       Stage st(p, "sort_downsweep1");
-      launch_downsweep<NM>(g1, s, raw, A, cm, kg, dsel, nb, tg.hi_bits, t1, perm1);
+      launch_downsweep<NM>(g1, s, raw, A, cm, kg, dsel, nb, t1, perm1);
     }
     tile_start = static_cast<uint32_t *>(p.tile_start.ensure((size_t)(ks + 1) * 4));
     hipLaunchKernelGGL(k_tile_start_1pass, dim3((unsigned)ceil_div(ks + 1, 256)), dim3(256), 0, s, t1, g1, (int)ks,
@@ -413,7 +506,7 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     ks = (int64_t)nhi * nlo;
     // pass 1: high digit over [0, n)
     uint32_t *t1 = static_cast<uint32_t *>(p.table1.ensure((size_t)nhi * g1 * 4));
-    DigitSel d1{tg.lo_bits, (uint32_t)(nhi - 1)};
+    const DigitSel d1 = digit_sel(kg, tg.lo_bits, tg.hi_bits);
     ChunkMap cm1{n, 0, nullptr, nullptr, g1, 0};
     {
       Stage st(p, "sort_upsweep1");
@@ -424,7 +517,7 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     exclusive_scan_u32(t1, t1, (int64_t)nhi * g1, p.scan_tmp, s);
     {
       Stage st(p, "sort_downsweep1");
-      launch_downsweep<NM>(g1, s, raw, A, cm1, kg, d1, nhi, tg.hi_bits, t1, perm1);
+      launch_downsweep<NM>(g1, s, raw, A, cm1, kg, d1, nhi, t1, perm1);
     }
     // segments of pass 2
     uint32_t *seg = static_cast<uint32_t *>(p.seginfo.ensure((size_t)(nhi + 1) * 3 * 4));
@@ -435,7 +528,7 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     exclusive_scan_u32(nchunks, chunk_start, nhi + 1, p.scan_tmp, s);
     const int64_t g2 = g1 + nhi;  // upper bound of pass-2 chunks
     uint32_t *t2 = static_cast<uint32_t *>(p.table2.ensure((size_t)g2 * nlo * 4));
-    DigitSel d2{0, (uint32_t)(nlo - 1)};
+    const DigitSel d2 = digit_sel(kg, 0, tg.lo_bits);
     ChunkMap cm2{n, nhi, seg_start, chunk_start, 0, nlo};
     Src<NM> pa{A, nullptr, false};
     {
@@ -453,7 +546,7 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     GENPK_LAUNCH_CHECK();
     {
       Stage st(p, "sort_downsweep2");
-      launch_downsweep<NM>(g2, s, pa, B, cm2, kg, d2, nlo, tg.lo_bits, t2, perm2);
+      launch_downsweep<NM>(g2, s, pa, B, cm2, kg, d2, nlo, t2, perm2);
     }
     sorted = B;
   }
