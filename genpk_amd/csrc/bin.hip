@@ -111,6 +111,26 @@ struct BinGeom {
   const int *q2bin;
   const int64_t *thr;
   const float *wtab;
+  float lq;  // bin(q) ~ lq * log2(q): binsperunit * ln(2) / 2
+};
+
+// Bin of q from LDS copies of the thresholds (thr32[b] = smallest q whose
+// bin is >= b): a candidate from log2 (off by at most one near a bin edge)
+// moved to the bin whose [thr32[b], thr32[b+1]) holds q.
+struct BinTrackLds {
+  int cur = -1, lo = 0, hi = 0;
+  __device__ __forceinline__ int of(int q, const int *thr32, int nrbins, float lq) {
+    if (cur < 0 || q < lo || q >= hi) {
+      int b = (int)(lq * __log2f((float)q));
+      b = min(max(b, 0), nrbins - 1);
+      while (b > 0 && q < thr32[b]) b--;
+      while (b < nrbins - 1 && q >= thr32[b + 1]) b++;
+      cur = b;
+      lo = thr32[b];
+      hi = thr32[b + 1];
+    }
+    return cur;
+  }
 };
 
 // Bin of q, tracked along a run of modes: the table is read only when q
@@ -144,8 +164,12 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
   const int n = rp.n;
   const int lgb = ilog2(B);
   C *stw = buf + (size_t)NF * n * B;  // twiddles staged in LDS
+  float *wt = reinterpret_cast<float *>(stw + n);  // deconvolution weights [d]
+  int *thr32 = reinterpret_cast<int *>(wt + g.d);  // bin thresholds [nrbins + 1]
   for (int i = threadIdx.x; i < n; i += BIN_THREADS) stw[i] = tw[i];
   for (int i = threadIdx.x; i < NS * g.nrbins; i += BIN_THREADS) bins[i] = 0;
+  for (int i = threadIdx.x; i < g.d; i += BIN_THREADS) wt[i] = g.wtab[i];
+  for (int i = threadIdx.x; i <= g.nrbins; i += BIN_THREADS) thr32[i] = (int)g.thr[i];
   const int64_t d = g.d, hz = g.hz;
   const int64_t estride = d * hz;
   const int64_t nblk = d * nkb;
@@ -213,16 +237,16 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
       const int64_t ky = kval(y, d);
       const double h = (kz == 0 || kz == d / 2) ? 1.0 : 2.0;
       const int64_t a = ky * ky + kz * kz;
-      const float wy = g.wtab[y], wz = g.wtab[kz];
+      const float wy = wt[y], wz = wt[kz];
       RunAcc r;
-      BinTrack bt;
+      BinTrackLds bt;
       const int e1 = min(n, e0 + per);
       for (int e = e0; e < e1; e++) {
         const int64_t kx = kval(e, d);
         const int64_t q = a + kx * kx;
         if (q == 0) continue;
-        const int b = bt.of(q, g);
-        const float gw = (g.wtab[e] * wy) * wz;   // the reference's ((wx*wy)*wz) in float
+        const int b = bt.of((int)q, thr32, g.nrbins, g.lq);
+        const float gw = (wt[e] * wy) * wz;   // the reference's ((wx*wy)*wz) in float
         double dc = (double)gw * (double)gw;
         dc *= dc;
         const C ma = buf[e * B + my_b];
@@ -334,6 +358,7 @@ BinTables &bin_tables(Plan &p, int nrbins) {
   std::vector<int> tab((size_t)qmax + 1);
   // binsperunit and the per-mode index exactly as powerspectrum.c:38,66
   const double binsperunit = (nrbins - 1) / log(sqrt(3) * d / 2.0);
+  bt->lq = (float)(binsperunit * log(2.0) / 2);
   tab[0] = -1;
   for (int64_t q = 1; q <= qmax; q++) {
     const double kk = sqrt((double)q);
@@ -417,7 +442,8 @@ static int fused_block(int prec, int64_t d, int64_t hz, int nf, int ns, int nrbi
   const int elems = prec == 4 ? BinCfg<float>::ELEMS : BinCfg<double>::ELEMS;
   int B = col_block(prec, d, hz);
   while (B >= 1) {
-    const int64_t lds = ((int64_t)nf * B + 1) * d * csize + (int64_t)((ns * nrbins + 1) & ~1) * 8;
+    const int64_t lds = ((int64_t)nf * B + 1) * d * csize + (int64_t)((ns * nrbins + 1) & ~1) * 8 + d * 4 +
+                        (nrbins + 1) * 4;
     if (lds <= LDS_LIMIT && d * B <= elems && BIN_THREADS % B == 0) return B;
     B /= 2;
   }
@@ -428,7 +454,8 @@ template <typename T, int NF>
 static size_t fused_lds(const Plan &a, int B, int nrbins) {
   using C = typename Cx<T>::C;
   constexpr int NS = NF == 1 ? 1 : 3;
-  return ((size_t)NF * B + 1) * a.d * sizeof(C) + (size_t)((NS * nrbins + 1) & ~1) * 8;
+  return ((size_t)NF * B + 1) * a.d * sizeof(C) + (size_t)((NS * nrbins + 1) & ~1) * 8 + (size_t)a.d * 4 +
+         (size_t)(nrbins + 1) * 4;
 }
 
 // Persistent grid of the fused kernel: as many workgroups as stay resident
@@ -449,7 +476,7 @@ static void launch_fused(Plan &a, Plan *b, const BinTables &bt, int B, bool keep
   const int nkb = (int)ceil_div(a.hz, B);
   const RadixPlan rp = make_radix_plan((int)a.d);
   const C *tw = (a.prec == 4) ? (const C *)a.fft.tw_d_f32.ptr : (const C *)a.fft.tw_d_f64.ptr;
-  BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), bt.thr.as<int64_t>(), a.wtab.as<float>()};
+  BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), bt.thr.as<int64_t>(), a.wtab.as<float>(), bt.lq};
   const size_t lds = fused_lds<T, NF>(a, B, bt.nrbins);
   Stage st(a, "xpass_bin");
   hipLaunchKernelGGL((k_xpass_bin<T, NF>), dim3(G), dim3(BIN_THREADS), lds, a.stream, a.field.as<C>(),
@@ -461,7 +488,7 @@ template <typename T, int NF>
 static void launch_stored(Plan &a, Plan *b, const BinTables &bt, double *part, int G) {
   using C = typename Cx<T>::C;
   constexpr int NS = NF == 1 ? 1 : 3;
-  BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), bt.thr.as<int64_t>(), a.wtab.as<float>()};
+  BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), bt.thr.as<int64_t>(), a.wtab.as<float>(), bt.lq};
   Stage st(a, "bin_stored");
   hipLaunchKernelGGL((k_bin_stored<T, NF>), dim3(G), dim3(BIN_THREADS), (size_t)NS * bt.nrbins * 8, a.stream,
                      a.field.as<C>(), b ? b->field.as<C>() : (const C *)nullptr, g, part);

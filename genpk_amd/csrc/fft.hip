@@ -50,6 +50,36 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
   const int tot = L * M;
   C v[MAXE];
   T zh[FROM_TILES ? MAXE : 1];
+  if (FROM_TILES && M == FFT_THREADS) {
+    // element q of this thread is row row0 + q, reals 2m and 2m + 1 with m =
+    // threadIdx.x: the row's part of the store offset is the same for the
+    // whole workgroup (scalar), the z part is the lane's own
+    const int z = 2 * (int)threadIdx.x;
+    const int tz = z >> fg.sz, lz = z & (fg.TZ - 1);
+    const int64_t lane_off = (int64_t)tz * fg.cm + lz;
+    // z-halo cell added to the first cell of a tile along z: local z = width
+    // of the previous tile, in its row store when that tile is partial
+    const int tzp = tz == 0 ? fg.ntz - 1 : tz - 1;
+    const int hw = min(fg.TZ, fg.d - tzp * fg.TZ);
+    const bool hrow = hw < fg.TZ;
+    const int64_t h_off = hrow ? (int64_t)tzp * fg.cm + hw : fg.zoff + (int64_t)tzp * fg.cz;
+    const int64_t h_tmul = hrow ? (int64_t)fg.ntz * fg.cm : (int64_t)fg.ntz * fg.cz;
+    const int64_t h_rmul = hrow ? fg.TZ : 1;
+#pragma unroll
+    for (int q = 0; q < MAXE; q++) {
+      v[q] = C{0, 0};
+      zh[FROM_TILES ? q : 0] = (T)0;
+      if (q < L && q < nl) {
+        const int64_t row = row0 + q;
+        const int x = (int)fdd.div((uint32_t)row), y = (int)(row - (int64_t)x * fg.d);
+        const int64_t txy = (int64_t)(x >> fg.sx) * fg.nty + (y >> fg.sy);
+        const int64_t r = (int64_t)(x & (fg.TX - 1)) * fg.HY + (y & (fg.TY - 1));
+        const int64_t at = txy * fg.ntz * fg.cm + r * fg.TZ + lane_off;
+        v[q] = C{tiles[at], tiles[at + 1]};
+        if (lz == 0) zh[FROM_TILES ? q : 0] = tiles[h_off + txy * h_tmul + r * h_rmul];
+      }
+    }
+  } else {
 #pragma unroll
   for (int q = 0; q < MAXE; q++) {
     const int i = threadIdx.x + q * FFT_THREADS;
@@ -72,6 +102,7 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
         }
       }
     }
+  }
   }
 #pragma unroll
   for (int q = 0; q < MAXE; q++) {

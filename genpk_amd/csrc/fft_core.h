@@ -208,11 +208,17 @@ __device__ __forceinline__ void fft_stage(C *buf, int nbatch, int lgb, int se, i
       const int k = j - jq * ns;
       const C *src = buf + b * sb + j * se;
       v[i][0] = src[0];
+      if (ns == 1) {
+        // first stage: every twiddle is 1
 #pragma unroll
-      for (int r = 1; r < R; r++) {
-        C t = tw[r * k * sp.twstep];
-        if (SIGN > 0) t.y = -t.y;
-        v[i][r] = cmul(src[r * nq * se], t);
+        for (int r = 1; r < R; r++) v[i][r] = src[r * nq * se];
+      } else {
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+          C t = tw[r * k * sp.twstep];
+          if (SIGN > 0) t.y = -t.y;
+          v[i][r] = cmul(src[r * nq * se], t);
+        }
       }
       dsto[i] = b * sb + (jq * ns * R + k) * se;
     }

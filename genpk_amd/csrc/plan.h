@@ -77,6 +77,7 @@ struct BinTables {
   DevBuf thr;       // int64 [nrbins+1]: smallest q >= 1 whose bin is >= b (qmax+1 past the end)
   DevBuf count;     // int32 [nrbins]
   DevBuf keffsum;   // double [nrbins], unnormalised sum of h*|k|
+  float lq = 0;     // bin(q) ~ lq * log2(q)
   bool counts_ready = false;
 };
 
