@@ -198,10 +198,6 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
       if (NF == 2) vb[NF == 2 ? q : 0] = ok ? fb[off + e * estride + b] : C{0, 0};
     }
   };
-  // one field: the next block's loads overlap this block's FFT and binning
-  // (two fields would hold twice the registers across the FFT and spill)
-  constexpr bool PREFETCH = NF == 1;
-  if (PREFETCH && (int64_t)blockIdx.x < nblk) load(blockIdx.x);
   for (int64_t v = blockIdx.x; v < nblk; v += gridDim.x) {
     const int64_t blk = xcd_block((uint32_t)v, (uint32_t)nblk);
     const int64_t y = blk / nkb;
@@ -209,7 +205,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
     const int64_t kz0 = (int64_t)kb * B;
     const int nb = (int)min((int64_t)B, hz - kz0);
     const int64_t off = y * hz + kz0;
-    if (!PREFETCH) load(v);
+    load(v);
     __syncthreads();  // the previous block's binning is done with buf
 #pragma unroll
     for (int q = 0; q < MAXE; q++) {
@@ -220,7 +216,6 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
       }
     }
     __syncthreads();
-    if (PREFETCH && v + gridDim.x < nblk) load(v + gridDim.x);
     lds_fft<-1, true, BinCfg<T>::MAXE>(buf, rp, B, lgb, B, 1, stw);
     if (NF == 2) lds_fft<-1, true, BinCfg<T>::MAXE>(buf + n * B, rp, B, lgb, B, 1, stw);
     if (keep) {
@@ -233,27 +228,47 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
       }
     }
     if (my_b < nb) {
-      const int64_t kz = kz0 + my_b;
-      const int64_t ky = kval(y, d);
+      const int kz = (int)(kz0 + my_b);
+      const int ky = (int)kval(y, d);
       const double h = (kz == 0 || kz == d / 2) ? 1.0 : 2.0;
-      const int64_t a = ky * ky + kz * kz;
+      const int a = ky * ky + kz * kz;
       const float wy = wt[y], wz = wt[kz];
       RunAcc r;
       BinTrackLds bt;
-      const int e1 = min(n, e0 + per);
-      for (int e = e0; e < e1; e++) {
-        const int64_t kx = kval(e, d);
-        const int64_t q = a + kx * kx;
-        if (q == 0) continue;
-        const int b = bt.of((int)q, thr32, g.nrbins, g.lq);
-        const float gw = (wt[e] * wy) * wz;   // the reference's ((wx*wy)*wz) in float
-        double dc = (double)gw * (double)gw;
-        dc *= dc;
-        const C ma = buf[e * B + my_b];
-        const C mb = NF == 2 ? buf[n * B + e * B + my_b] : ma;
-        double c[3];
-        mode_terms<C, NF>(ma, mb, h, dc, c);
-        run_add<NS>(r, b, c, bins, g.nrbins);
+      // this thread's run of at most MAXE modes along x (per <= MAXE as
+      // n * B <= BIN_THREADS * MAXE); with one field all its LDS reads are
+      // issued first (two fields would not fit the registers)
+      constexpr bool BATCH = NF == 1;
+      C ma[BATCH ? MAXE : 1];
+      float wx[BATCH ? MAXE : 1];
+      if (BATCH) {
+#pragma unroll
+        for (int j = 0; j < MAXE; j++) {
+          const int e = e0 + j;
+          if (j < per && e < n) {
+            ma[BATCH ? j : 0] = buf[e * B + my_b];
+            wx[BATCH ? j : 0] = wt[e];
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < MAXE; j++) {
+        const int e = e0 + j;
+        if (j < per && e < n) {
+          const int kx = e <= d / 2 ? e : e - (int)d;
+          const int q = a + kx * kx;
+          if (q != 0) {
+            const int b = bt.of(q, thr32, g.nrbins, g.lq);
+            const float gw = ((BATCH ? wx[BATCH ? j : 0] : wt[e]) * wy) * wz;  // the reference's ((wx*wy)*wz) in float
+            double dc = (double)gw * (double)gw;
+            dc *= dc;
+            const C fa_m = BATCH ? ma[BATCH ? j : 0] : buf[e * B + my_b];
+            const C fb_m = NF == 2 ? buf[n * B + e * B + my_b] : fa_m;
+            double c[3];
+            mode_terms<C, NF>(fa_m, fb_m, h, dc, c);
+            run_add<NS>(r, b, c, bins, g.nrbins);
+          }
+        }
       }
       run_flush<NS>(r, bins, g.nrbins);
     }
