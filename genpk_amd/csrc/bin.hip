@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
                                                             typename Cx<T>::C *__restrict__ fb, BinGeom g, int B,
                                                             int nkb, RadixPlan rp,
                                                             const typename Cx<T>::C *__restrict__ tw, int keep,
-                                                            double *__restrict__ part) {
+                                                            int pp, double *__restrict__ part) {
   using C = typename Cx<T>::C;
   constexpr int NS = NF == 1 ? 1 : 3;
   extern __shared__ unsigned char lds_raw[];
@@ -163,7 +163,9 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
   C *buf = reinterpret_cast<C *>(bins + ((NS * g.nrbins + 1) & ~1));  // [NF][n][B], 16-byte aligned
   const int n = rp.n;
   const int lgb = ilog2(B);
-  C *stw = buf + (size_t)NF * n * B;  // twiddles staged in LDS
+  // pp (one field): a second buffer lets the FFT stages alternate (one barrier each)
+  C *buf2 = buf + (size_t)NF * n * B;
+  C *stw = buf2 + (pp ? (size_t)n * B : 0);  // twiddles staged in LDS
   float *wt = reinterpret_cast<float *>(stw + n);  // deconvolution weights [d]
   int *thr32 = reinterpret_cast<int *>(wt + g.d);  // bin thresholds [nrbins + 1]
   for (int i = threadIdx.x; i < n; i += BIN_THREADS) stw[i] = tw[i];
@@ -216,14 +218,19 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
       }
     }
     __syncthreads();
-    lds_fft<-1, true, BinCfg<T>::MAXE>(buf, rp, B, lgb, B, 1, stw);
-    if (NF == 2) lds_fft<-1, true, BinCfg<T>::MAXE>(buf + n * B, rp, B, lgb, B, 1, stw);
+    C *res = buf;
+    if (NF == 1 && pp) {
+      res = lds_fft_pp<-1, true, BinCfg<T>::MAXE>(buf, buf2, rp, B, lgb, B, 1, stw);
+    } else {
+      lds_fft<-1, true, BinCfg<T>::MAXE>(buf, rp, B, lgb, B, 1, stw);
+      if (NF == 2) lds_fft<-1, true, BinCfg<T>::MAXE>(buf + n * B, rp, B, lgb, B, 1, stw);
+    }
     if (keep) {
       for (int i = threadIdx.x; i < n * B; i += BIN_THREADS) {
         const int e = i >> lgb, b = i & (B - 1);
         if (b < nb) {
-          fa[off + e * estride + b] = buf[i];
-          if (NF == 2) fb[off + e * estride + b] = buf[n * B + i];
+          fa[off + e * estride + b] = res[i];
+          if (NF == 2) fb[off + e * estride + b] = res[n * B + i];
         }
       }
     }
@@ -246,7 +253,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
         for (int j = 0; j < MAXE; j++) {
           const int e = e0 + j;
           if (j < per && e < n) {
-            ma[BATCH ? j : 0] = buf[e * B + my_b];
+            ma[BATCH ? j : 0] = res[e * B + my_b];
             wx[BATCH ? j : 0] = wt[e];
           }
         }
@@ -262,8 +269,8 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
             const float gw = ((BATCH ? wx[BATCH ? j : 0] : wt[e]) * wy) * wz;  // the reference's ((wx*wy)*wz) in float
             double dc = (double)gw * (double)gw;
             dc *= dc;
-            const C fa_m = BATCH ? ma[BATCH ? j : 0] : buf[e * B + my_b];
-            const C fb_m = NF == 2 ? buf[n * B + e * B + my_b] : fa_m;
+            const C fa_m = BATCH ? ma[BATCH ? j : 0] : res[e * B + my_b];
+            const C fb_m = NF == 2 ? res[n * B + e * B + my_b] : fa_m;
             double c[3];
             mode_terms<C, NF>(fa_m, fb_m, h, dc, c);
             run_add<NS>(r, b, c, bins, g.nrbins);
@@ -466,36 +473,36 @@ static int fused_block(int prec, int64_t d, int64_t hz, int nf, int ns, int nrbi
 }
 
 template <typename T, int NF>
-static size_t fused_lds(const Plan &a, int B, int nrbins) {
+static size_t fused_lds(const Plan &a, int B, int nrbins, bool pp) {
   using C = typename Cx<T>::C;
   constexpr int NS = NF == 1 ? 1 : 3;
-  return ((size_t)NF * B + 1) * a.d * sizeof(C) + (size_t)((NS * nrbins + 1) & ~1) * 8 + (size_t)a.d * 4 +
-         (size_t)(nrbins + 1) * 4;
+  return ((size_t)NF * B + (pp ? B : 0) + 1) * a.d * sizeof(C) + (size_t)((NS * nrbins + 1) & ~1) * 8 +
+         (size_t)a.d * 4 + (size_t)(nrbins + 1) * 4;
 }
 
 // Persistent grid of the fused kernel: as many workgroups as stay resident
 // (occupancy of this instance x CUs), at most one per column block.
 template <typename T, int NF>
-static int fused_grid(const Plan &a, int B, int nrbins) {
+static int fused_grid(const Plan &a, int B, int nrbins, bool pp) {
   int per_cu = 0;
   GENPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_xpass_bin<T, NF>),
-                                                          BIN_THREADS, fused_lds<T, NF>(a, B, nrbins)));
+                                                          BIN_THREADS, fused_lds<T, NF>(a, B, nrbins, pp)));
   const int64_t nblk = a.d * ceil_div(a.hz, B);
   return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)std::max(per_cu, 1) * num_cus(), nblk));
 }
 
 template <typename T, int NF>
-static void launch_fused(Plan &a, Plan *b, const BinTables &bt, int B, bool keep, double *part, int G) {
+static void launch_fused(Plan &a, Plan *b, const BinTables &bt, int B, bool keep, bool pp, double *part, int G) {
   using C = typename Cx<T>::C;
   constexpr int NS = NF == 1 ? 1 : 3;
   const int nkb = (int)ceil_div(a.hz, B);
   const RadixPlan rp = make_radix_plan((int)a.d);
   const C *tw = (a.prec == 4) ? (const C *)a.fft.tw_d_f32.ptr : (const C *)a.fft.tw_d_f64.ptr;
   BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), bt.thr.as<int64_t>(), a.wtab.as<float>(), bt.lq};
-  const size_t lds = fused_lds<T, NF>(a, B, bt.nrbins);
+  const size_t lds = fused_lds<T, NF>(a, B, bt.nrbins, pp);
   Stage st(a, "xpass_bin");
   hipLaunchKernelGGL((k_xpass_bin<T, NF>), dim3(G), dim3(BIN_THREADS), lds, a.stream, a.field.as<C>(),
-                     b ? b->field.as<C>() : (C *)nullptr, g, B, nkb, rp, tw, keep ? 1 : 0, part);
+                     b ? b->field.as<C>() : (C *)nullptr, g, B, nkb, rp, tw, keep ? 1 : 0, pp ? 1 : 0, part);
   GENPK_LAUNCH_CHECK();
 }
 
@@ -525,15 +532,20 @@ void bin_spectra(Plan &a, Plan *b, int nrbins, bool fused_xpass, double *pow_aa,
                   "fused binning needs fields after the z and y passes");
     const int B = fused_block(a.prec, a.d, a.hz, nf, ns, nrbins);
     GENPK_REQUIRE(B > 0, "fused x-pass does not fit in LDS for this size");
+    // one field: ping-pong FFT buffers where the LDS has room for them
+    const bool pp = !b && (a.prec == 4 ? fused_lds<float, 1>(a, B, nrbins, true)
+                                       : fused_lds<double, 1>(a, B, nrbins, true)) <= (size_t)LDS_LIMIT;
     if (a.prec == 4)
-      G = b ? fused_grid<float, 2>(a, B, nrbins) : fused_grid<float, 1>(a, B, nrbins);
+      G = b ? fused_grid<float, 2>(a, B, nrbins, false) : fused_grid<float, 1>(a, B, nrbins, pp);
     else
-      G = b ? fused_grid<double, 2>(a, B, nrbins) : fused_grid<double, 1>(a, B, nrbins);
+      G = b ? fused_grid<double, 2>(a, B, nrbins, false) : fused_grid<double, 1>(a, B, nrbins, pp);
     part = static_cast<double *>(a.bin_part.ensure((size_t)G * ns * nrbins * 8));
     if (a.prec == 4) {
-      if (b) launch_fused<float, 2>(a, b, bt, B, false, part, G); else launch_fused<float, 1>(a, b, bt, B, false, part, G);
+      if (b) launch_fused<float, 2>(a, b, bt, B, false, false, part, G);
+      else launch_fused<float, 1>(a, b, bt, B, false, pp, part, G);
     } else {
-      if (b) launch_fused<double, 2>(a, b, bt, B, false, part, G); else launch_fused<double, 1>(a, b, bt, B, false, part, G);
+      if (b) launch_fused<double, 2>(a, b, bt, B, false, false, part, G);
+      else launch_fused<double, 1>(a, b, bt, B, false, pp, part, G);
     }
     // the field keeps the forward state after the z and y passes (an adjoint can start from it)
     a.fstate = FieldState::YDone;

@@ -181,12 +181,14 @@ inline bool fft_length_ok(int64_t n) {
 }
 
 // One Stockham stage of radix R on nbatch transforms held in LDS, element
-// e of transform b at buf[e*se + b*sb].  MAXE bounds the elements a thread
-// owns per stage.  BFAST: the batch index varies fastest across threads
-// (interleaved layout, nbatch = 1 << lgb); otherwise the element index does.
-template <int R, int SIGN, bool BFAST, int MAXE, typename C>
-__device__ __forceinline__ void fft_stage(C *buf, int nbatch, int lgb, int se, int sb, const StagePlan &sp,
-                                          const C *__restrict__ tw) {
+// e of transform b at in[e*se + b*sb], results to out (same layout).  MAXE
+// bounds the elements a thread owns per stage.  BFAST: the batch index
+// varies fastest across threads (interleaved layout, nbatch = 1 << lgb);
+// otherwise the element index does.  In place (in == out) all reads pass a
+// barrier before the writes; out of place (PP) one barrier ends the stage.
+template <int R, int SIGN, bool BFAST, int MAXE, bool PP, typename C>
+__device__ __forceinline__ void fft_stage(const C *in, C *out, int nbatch, int lgb, int se, int sb,
+                                          const StagePlan &sp, const C *__restrict__ tw) {
   constexpr int MAXB = (MAXE + R - 1) / R;
   const int nq = sp.nq, ns = sp.ns;
   const int nbf = nbatch * nq;
@@ -206,7 +208,7 @@ __device__ __forceinline__ void fft_stage(C *buf, int nbatch, int lgb, int se, i
       }
       const int jq = (int)sp.fns.div((uint32_t)j);
       const int k = j - jq * ns;
-      const C *src = buf + b * sb + j * se;
+      const C *src = in + b * sb + j * se;
       v[i][0] = src[0];
       if (ns == 1) {
         // first stage: every twiddle is 1
@@ -223,13 +225,13 @@ __device__ __forceinline__ void fft_stage(C *buf, int nbatch, int lgb, int se, i
       dsto[i] = b * sb + (jq * ns * R + k) * se;
     }
   }
-  __syncthreads();
+  if (!PP) __syncthreads();
 #pragma unroll
   for (int i = 0; i < MAXB; i++) {
     const int u = threadIdx.x + i * blockDim.x;
     if (u < nbf) {
       Dft<R, SIGN, C>::run(v[i]);
-      C *dst = buf + dsto[i];
+      C *dst = out + dsto[i];
 #pragma unroll
       for (int r = 0; r < R; r++) dst[r * ns * se] = v[i][r];
     }
@@ -237,21 +239,40 @@ __device__ __forceinline__ void fft_stage(C *buf, int nbatch, int lgb, int se, i
   __syncthreads();
 }
 
-// Full transform; the caller has filled buf and synchronised.
+template <int SIGN, bool BFAST, int MAXE, bool PP, typename C>
+__device__ __forceinline__ void fft_stage_r(int R, const C *in, C *out, int nbatch, int lgb, int se, int sb,
+                                            const StagePlan &sp, const C *__restrict__ tw) {
+  switch (R) {
+    case 8: fft_stage<8, SIGN, BFAST, MAXE, PP>(in, out, nbatch, lgb, se, sb, sp, tw); break;
+    case 4: fft_stage<4, SIGN, BFAST, MAXE, PP>(in, out, nbatch, lgb, se, sb, sp, tw); break;
+    case 2: fft_stage<2, SIGN, BFAST, MAXE, PP>(in, out, nbatch, lgb, se, sb, sp, tw); break;
+    case 3: fft_stage<3, SIGN, BFAST, MAXE, PP>(in, out, nbatch, lgb, se, sb, sp, tw); break;
+    case 5: fft_stage<5, SIGN, BFAST, MAXE, PP>(in, out, nbatch, lgb, se, sb, sp, tw); break;
+    default: break;
+  }
+}
+
+// Full transform in place; the caller has filled buf and synchronised.
 template <int SIGN, bool BFAST, int MAXE, typename C>
 __device__ __forceinline__ void lds_fft(C *buf, const RadixPlan &rp, int nbatch, int lgb, int se, int sb,
-                        const C *__restrict__ tw) {
+                                        const C *__restrict__ tw) {
+  for (int s = 0; s < rp.nst; s++)
+    fft_stage_r<SIGN, BFAST, MAXE, false>(rp.st[s].R, buf, buf, nbatch, lgb, se, sb, rp.st[s], tw);
+}
+
+// Full transform alternating between buffers a and b (one barrier per
+// stage); returns the buffer that holds the result.  The caller has filled a
+// and synchronised.
+template <int SIGN, bool BFAST, int MAXE, typename C>
+__device__ __forceinline__ C *lds_fft_pp(C *a, C *b, const RadixPlan &rp, int nbatch, int lgb, int se, int sb,
+                                         const C *__restrict__ tw) {
   for (int s = 0; s < rp.nst; s++) {
-    const StagePlan &sp = rp.st[s];
-    switch (sp.R) {
-      case 8: fft_stage<8, SIGN, BFAST, MAXE>(buf, nbatch, lgb, se, sb, sp, tw); break;
-      case 4: fft_stage<4, SIGN, BFAST, MAXE>(buf, nbatch, lgb, se, sb, sp, tw); break;
-      case 2: fft_stage<2, SIGN, BFAST, MAXE>(buf, nbatch, lgb, se, sb, sp, tw); break;
-      case 3: fft_stage<3, SIGN, BFAST, MAXE>(buf, nbatch, lgb, se, sb, sp, tw); break;
-      case 5: fft_stage<5, SIGN, BFAST, MAXE>(buf, nbatch, lgb, se, sb, sp, tw); break;
-      default: break;
-    }
+    fft_stage_r<SIGN, BFAST, MAXE, true>(rp.st[s].R, a, b, nbatch, lgb, se, sb, rp.st[s], tw);
+    C *t = a;
+    a = b;
+    b = t;
   }
+  return a;
 }
 
 __host__ __device__ inline int ilog2(int64_t v) {
