@@ -119,14 +119,33 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
   __syncthreads();
   lds_fft<-1, false, MAXE>(buf, rp, L, 0, 1, M, stw);
   // X_k = (Z_k + conj Z_{M-k})/2 - i/2 * w^k (Z_k - conj Z_{M-k}),  w = exp(-2 pi i / d)
+  auto split = [&](C zk, C zc, C w) {
+    const C e = C{(zk.x + zc.x) * (T)0.5, (zk.y + zc.y) * (T)0.5};
+    const C o = C{(zk.y - zc.y) * (T)0.5, -(zk.x - zc.x) * (T)0.5};  // (zk - zc) / (2i)
+    return cadd(e, cmul(o, w));
+  };
+  if (M == FFT_THREADS) {
+    // output k = threadIdx.x of every row (L <= MAXE rows), and k = M by thread 0
+    const int k = threadIdx.x;
+    const C w = sxw[k], wm = sxw[M];
+#pragma unroll
+    for (int l = 0; l < MAXE; l++) {
+      if (l < nl) {
+        const C zk = buf[l * M + k];
+        const C zc = cconj(buf[l * M + (k == 0 ? 0 : M - k)]);
+        C *row = reinterpret_cast<C *>(field + (row0 + l) * rowlen);
+        row[k] = split(zk, zc, w);
+        if (k == 0) row[M] = split(zk, zc, wm);
+      }
+    }
+    return;
+  }
   const int nout = M + 1;
   for (int i = threadIdx.x; i < nl * nout; i += FFT_THREADS) {
     const int l = (int)fdM1.div((uint32_t)i), k = i - l * nout;
     const C zk = buf[l * M + (k == M ? 0 : k)];
     const C zc = cconj(buf[l * M + (k == 0 ? 0 : M - k)]);
-    const C e = C{(zk.x + zc.x) * (T)0.5, (zk.y + zc.y) * (T)0.5};
-    const C o = C{(zk.y - zc.y) * (T)0.5, -(zk.x - zc.x) * (T)0.5};  // (zk - zc) / (2i)
-    const C x = cadd(e, cmul(o, sxw[k]));
+    const C x = split(zk, zc, sxw[k]);
     T *row = field + (row0 + l) * rowlen;
     row[2 * k] = x.x;
     row[2 * k + 1] = x.y;
