@@ -47,8 +47,10 @@ struct DepParams {
 };
 
 // Records are RW floats: x, y, z, then the mass when NM (FP64 LDS
-// accumulation).
-template <typename Real, int NM, int RW>
+// accumulation).  STD: tiles of 16 x 32 x 32 cells (every grid of d >= 32),
+// whose strides are constants, so the 8 corner offsets fold into the LDS
+// instructions' immediate offsets.
+template <typename Real, int NM, int RW, bool STD>
 __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict__ rec,
                                                          const uint32_t *__restrict__ tile_start,
                                                          const uint32_t *__restrict__ item_start,
@@ -70,8 +72,9 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
   const int tz = (int)(k % ntz);
   const int ty = (int)((k / ntz) % nty);
   const int tx = (int)(k / ((int64_t)ntz * nty));
-  const int ox = tx << P.kg.sx, oy = ty << P.kg.sy, oz = tz << P.kg.sz;
-  const int sy = P.HZ, sx = P.HY * P.HZ;
+  const int TX = STD ? 16 : P.kg.TX, TY = STD ? 32 : P.kg.TY, TZ = STD ? 32 : P.kg.TZ;
+  const int ox = tx * TX, oy = ty * TY, oz = tz * TZ;
+  const int sy = STD ? 33 : P.HZ, sx = STD ? 33 * 33 : P.HY * P.HZ;
   float r[UNR][RW], nx[UNR][RW];
 #pragma unroll
   for (int u = 0; u < UNR; u++) {
@@ -103,8 +106,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
       const int lx = cell_of(r[u][0], P.kg.units, P.kg.d, fx) - ox;
       const int ly = cell_of(r[u][1], P.kg.units, P.kg.d, fy) - oy;
       const int lz = cell_of(r[u][2], P.kg.units, P.kg.d, fz) - oz;
-      if ((unsigned)lx >= (unsigned)P.kg.TX || (unsigned)ly >= (unsigned)P.kg.TY ||
-          (unsigned)lz >= (unsigned)P.kg.TZ)
+      if ((unsigned)lx >= (unsigned)TX || (unsigned)ly >= (unsigned)TY || (unsigned)lz >= (unsigned)TZ)
         continue;  // cannot happen for records of this tile; guards the LDS bounds
       const double gx = 1.0 - fx, gy = 1.0 - fy, gz = 1.0 - fz;
       const double wxy[4] = {gx * gy, fx * gy, gx * fy, fx * fy};
@@ -257,12 +259,20 @@ void deposit_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass) {
   const size_t lds = (size_t)tg.cells * 8;
   {
     Stage st(p, "deposit_tiles");
-    if (p.prec == 4)
-      hipLaunchKernelGGL((k_deposit<float, NM, RW>), dim3((unsigned)max_items), dim3(DEP_THREADS), lds, s, sorted,
-                         tile_start, item_start, part_start, P, p.tiles.as<float>(), partials);
+    const bool std_tiles = tg.TX == 16 && tg.TY == 32 && tg.TZ == 32;
+    const dim3 grid((unsigned)max_items), block(DEP_THREADS);
+    if (p.prec == 4 && std_tiles)
+      hipLaunchKernelGGL((k_deposit<float, NM, RW, true>), grid, block, lds, s, sorted, tile_start, item_start,
+                         part_start, P, p.tiles.as<float>(), partials);
+    else if (p.prec == 4)
+      hipLaunchKernelGGL((k_deposit<float, NM, RW, false>), grid, block, lds, s, sorted, tile_start, item_start,
+                         part_start, P, p.tiles.as<float>(), partials);
+    else if (std_tiles)
+      hipLaunchKernelGGL((k_deposit<double, NM, RW, true>), grid, block, lds, s, sorted, tile_start, item_start,
+                         part_start, P, p.tiles.as<double>(), partials);
     else
-      hipLaunchKernelGGL((k_deposit<double, NM, RW>), dim3((unsigned)max_items), dim3(DEP_THREADS), lds, s, sorted,
-                         tile_start, item_start, part_start, P, p.tiles.as<double>(), partials);
+      hipLaunchKernelGGL((k_deposit<double, NM, RW, false>), grid, block, lds, s, sorted, tile_start, item_start,
+                         part_start, P, p.tiles.as<double>(), partials);
     GENPK_LAUNCH_CHECK();
   }
   {
