@@ -1,6 +1,6 @@
 // gen-pk: matter power spectra of Gadget snapshots on the MI355X, with the
 // reference driver's options, console output and output files
-// (gen-pk.cpp:74-360, read_fieldize.cpp:18-97, utils.cpp:7-72).
+// (gen-pk.cpp:74-360, read_fieldize.cpp:18-340, utils.cpp:7-72).
 //
 //   gen-pk -i snap -o outdir            auto spectrum of every particle type: PK-<type>-<snap>
 //   gen-pk -i snap -j other -o outdir   cross spectrum of each type between two snapshots: PX-<type>-<snap>
@@ -8,14 +8,20 @@
 //   -s 1                                 add the stars (type 4) to the baryons (type 0)
 //   -p 32|64                             grid/FFT precision (default 64, the reference's double)
 //
-// Particles are read on the host in chunks of at most 2^26 (as the reference
-// does), copied to the device and deposited into a plan; the fused
-// transform + binning runs on the device.  The total mass of a species is
-// the sum of its particle masses (header mass x count, or the MASS block)
-// plus 1 per read of a species, the reference's convention
-// (read_fieldize.cpp:94).  HDF5 and bigfile snapshots are not supported.
+// Snapshots are Gadget format I/II file sets, or Gadget-HDF5 file sets when
+// built with the HDF5 library (GENPK_HDF5).  Particles are read on the host
+// (Gadget: chunks of at most 2^26, as the reference; HDF5: a file at a time),
+// copied to the device in chunks of at most 2^26 and deposited into a plan;
+// the fused transform + binning runs on the device.  The total mass of a
+// species is the sum of its particle masses (header mass x count, or the
+// MASS block / Masses dataset), plus 1 per read of a species from a Gadget
+// set (read_fieldize.cpp:94); HDF5 reads add nothing (:320-331).  Bigfile
+// snapshots are not supported.
 #include "../../include/genpk_amd.h"
 #include "gadget.h"
+#ifdef GENPK_HDF5
+#include "hdf5snap.h"
+#endif
 
 #include <hip/hip_runtime.h>
 
@@ -23,6 +29,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <memory>
 #include <string>
 #include <unistd.h>
 #include <vector>
@@ -129,6 +136,67 @@ int read_fieldize(genpk_plan *plan, const gadget::Snapshot &snap, int type, doub
   return 0;
 }
 
+// Deposit n particles held on the host (masses null: constant mass).
+void deposit_host(genpk_plan *plan, double box, int64_t n, const float *pos, const float *masses, double mass,
+                  Staging &st, hipStream_t s) {
+  for (int64_t at = 0; at < n; at += CHUNK) {
+    const int64_t parts = std::min(CHUNK, n - at);
+    st.ensure(parts);
+    hip_ok(hipMemcpyAsync(st.pos, pos + 3 * at, (size_t)parts * 12, hipMemcpyHostToDevice, s), "copy positions");
+    if (masses)
+      hip_ok(hipMemcpyAsync(st.mass, masses + at, (size_t)parts * 4, hipMemcpyHostToDevice, s), "copy masses");
+    ok(genpk_deposit(plan, box, parts, st.pos, masses ? st.mass : nullptr, mass), "deposit");
+    hip_ok(hipStreamSynchronize(s), "synchronize");
+  }
+}
+
+#ifdef GENPK_HDF5
+// read_fieldize_hdf5 (read_fieldize.cpp:258-339) over the files of a set:
+// every file's particles of `type`; per file the masses summed in double in
+// file order, then + MassTable x count.  A file that cannot be read is
+// skipped, as the reference's caller does (gen-pk.cpp:213-216).
+void read_fieldize_hdf5(genpk_plan *plan, const std::vector<std::string> &files, int type, double box,
+                        double *total_mass, Staging &st, hipStream_t s) {
+  hdf5snap::FileTypeData d;
+  for (size_t fileno = 0; fileno < files.size(); fileno++) {
+    if (hdf5snap::read_type(files[fileno], type, (int)fileno, d) < 0 || d.n == 0) continue;
+    const bool per_particle = d.mass_table == 0;
+    if (per_particle) {
+      double this_file = 0;
+      for (int64_t i = 0; i < d.n; i++) this_file += d.masses[(size_t)i];
+      *total_mass += this_file;
+    }
+    *total_mass += d.mass_table * d.n;
+    deposit_host(plan, box, d.n, d.pos.data(), per_particle ? d.masses.data() : nullptr, d.mass_table, st, s);
+  }
+}
+#endif
+
+// A snapshot: a Gadget set or an HDF5 set.
+struct Source {
+  std::unique_ptr<gadget::Snapshot> gadget;
+  std::vector<std::string> hdf5;
+  explicit Source(const std::string &name) {
+#ifdef GENPK_HDF5
+    hdf5 = hdf5snap::find_set(name);
+    if (!hdf5.empty()) return;
+#endif
+    gadget = std::make_unique<gadget::Snapshot>(name);
+  }
+  bool is_hdf5() const { return !hdf5.empty(); }
+  // Gadget: read_fieldize, returning 1 when the snapshot holds none of the
+  // type; HDF5: read_fieldize_hdf5 over the files, returning 0.
+  int read(genpk_plan *plan, int type, double box, double *total_mass, Staging &st, hipStream_t s) const {
+#ifdef GENPK_HDF5
+    if (is_hdf5()) {
+      read_fieldize_hdf5(plan, hdf5, type, box, total_mass, st, s);
+      return 0;
+    }
+#endif
+    return read_fieldize(plan, *gadget, type, box, total_mass, st, s);
+  }
+};
+
 struct Spectrum {
   int nrbins;
   double *dpow_aa, *dpow_ab, *dkeff;
@@ -188,17 +256,39 @@ int main(int argc, char *argv[]) {
     return 0;
   }
   try {
-    gadget::Snapshot snap(infiles);
-    const gadget::Header &h = snap.header();
+    const Source snap(infiles);
     int64_t npart_total[gadget::N_TYPE];
-    for (int t = 0; t < gadget::N_TYPE; t++) npart_total[t] = snap.npart(t);
-    const double box = h.box;
-    std::printf("Boxsize=%g, ", box);
-    std::printf("NPart=(%g,%g,%g,%g,%g,%g)**3\n", std::cbrt((double)npart_total[0]), std::cbrt((double)npart_total[1]),
-                std::cbrt((double)npart_total[2]), std::cbrt((double)npart_total[3]),
-                std::cbrt((double)npart_total[4]), std::cbrt((double)npart_total[5]));
-    std::printf("Masses=[%g %g %g ]\n", h.mass[0], h.mass[1], h.mass[2]);
-    std::printf("redshift=%g, \xce\xa9_M=%g\n", h.redshift, h.omega0);
+    double box = 0;
+    if (snap.is_hdf5()) {
+#ifdef GENPK_HDF5
+      // load_hdf5_header (read_fieldize.cpp:209-254) and its console lines
+      std::fprintf(stderr, "Found %lu hdf5 files in snapshot\n", (unsigned long)snap.hdf5.size());
+      hdf5snap::Header h;
+      if (!hdf5snap::load_header(snap.hdf5[0], h)) {
+        std::fprintf(stderr, "Could not load header\n");
+        return 1;
+      }
+      for (int t = 0; t < gadget::N_TYPE; t++) npart_total[t] = h.npart_total[t];
+      box = h.box;
+      std::printf("NumPart=[%ld,%ld,%ld,%ld,%ld,%ld], ", (long)npart_total[0], (long)npart_total[1],
+                  (long)npart_total[2], (long)npart_total[3], (long)npart_total[4], (long)npart_total[5]);
+      std::printf("Masses=[%g %g %g %g %g %g], ", h.mass[0], h.mass[1], h.mass[2], h.mass[3], h.mass[4], h.mass[5]);
+      std::printf("Redshift=%g, \xce\xa9_M=%g \xce\xa9_L=%g\n", h.redshift, h.omega0, h.omegal);
+      std::printf("Expansion factor = %f\n", h.time);
+      std::printf("Hubble = %g Box=%g \n", h.h100, h.box);
+#endif
+    } else {
+      const gadget::Header &h = snap.gadget->header();
+      for (int t = 0; t < gadget::N_TYPE; t++) npart_total[t] = snap.gadget->npart(t);
+      box = h.box;
+      std::printf("Boxsize=%g, ", box);
+      std::printf("NPart=(%g,%g,%g,%g,%g,%g)**3\n", std::cbrt((double)npart_total[0]),
+                  std::cbrt((double)npart_total[1]), std::cbrt((double)npart_total[2]),
+                  std::cbrt((double)npart_total[3]), std::cbrt((double)npart_total[4]),
+                  std::cbrt((double)npart_total[5]));
+      std::printf("Masses=[%g %g %g ]\n", h.mass[0], h.mass[1], h.mass[2]);
+      std::printf("redshift=%g, \xce\xa9_M=%g\n", h.redshift, h.omega0);
+    }
     // grid: max over types of min(2*nexttwo((int)cbrt(N)), FIELD_DIMS) (gen-pk.cpp:169-173)
     int64_t field_dims = 0;
     for (int t = 0; t < gadget::N_TYPE; t++) {
@@ -224,15 +314,21 @@ int main(int argc, char *argv[]) {
         if (npart_total[type] == 0) continue;
         ok(genpk_clear(a), "clear");
         double total_mass = 0;
-        read_fieldize(a, snap, type, box, &total_mass, st, s);
-        if (type == BARYON_TYPE && stars_are_baryons) read_fieldize(a, snap, STARS_TYPE, box, &total_mass, st, s);
+        snap.read(a, type, box, &total_mass, st, s);
+        if (type == BARYON_TYPE && stars_are_baryons) snap.read(a, STARS_TYPE, box, &total_mass, st, s);
         std::printf("total_mass in type %d = %g\n", type, total_mass);
         std::fflush(stdout);
         sp.run(a, nullptr, total_mass, total_mass, s);
         sp.write(outdir + "/PK-" + type_str(type) + "-" + base);
       }
     } else if (!jinfiles.empty()) {
-      gadget::Snapshot snap2(jinfiles);
+      const Source snap2(jinfiles);
+      // With an HDF5 second set the reference crosses type 1 of the two sets
+      // under every type's name (gen-pk.cpp:271-288); so does this driver
+      // (the reference reads nothing for the first set when that one is a
+      // Gadget set).  An HDF5 first set with a Gadget second set reads each
+      // type from both (the reference dereferences a missing snapshot there).
+      const bool dm_only = snap2.is_hdf5();
       ok(genpk_plan_create(&b, field_dims, precision, s), "plan");
       for (int type = 0; type < gadget::N_TYPE; type++) {
         if (npart_total[type] == 0) continue;
@@ -240,8 +336,9 @@ int main(int argc, char *argv[]) {
         ok(genpk_clear(b), "clear");
         std::fprintf(stderr, "Reading...\n");
         double total_mass = 0, total_mass2 = 0;
-        if (read_fieldize(a, snap, type, box, &total_mass, st, s)) continue;
-        if (read_fieldize(b, snap2, type, box, &total_mass2, st, s)) continue;
+        const int rt = dm_only ? 1 : type;
+        if (snap.read(a, rt, box, &total_mass, st, s)) continue;
+        if (snap2.read(b, rt, box, &total_mass2, st, s)) continue;
         sp.run(a, b, total_mass, total_mass2, s);
         sp.write(outdir + "/PX-" + type_str(type) + "-" + base);
       }
@@ -254,8 +351,8 @@ int main(int argc, char *argv[]) {
       ok(genpk_clear(a), "clear");
       ok(genpk_clear(b), "clear");
       double total_mass = 0, total_mass2 = 0;
-      read_fieldize(a, snap, 1, box, &total_mass, st, s);
-      read_fieldize(b, snap, crosstype, box, &total_mass2, st, s);
+      snap.read(a, 1, box, &total_mass, st, s);
+      snap.read(b, crosstype, box, &total_mass2, st, s);
       sp.run(a, b, total_mass, total_mass2, s);
       sp.write(outdir + "/PK-DMx" + type_str(crosstype) + "-" + base);
     }

@@ -187,3 +187,98 @@ def test_cli_fp32_and_usage(gpu, tmp_path):
     assert "Usage" in r.stderr
     r = run(["-i", str(tmp_path / "missing"), "-o", str(out)], ok=False)
     assert r.returncode == 1
+
+
+# ---- Gadget-HDF5 sets (read_fieldize.cpp:99-340; gen-pk.cpp:117-134, 206-216,
+# 271-288, 316-328): no +1 in the total mass; -j with an HDF5 second set
+# crosses type 1 of both sets under every type's name.
+
+import hdf5_util as h5u  # noqa: E402
+
+
+def _hdf5_field(box, d, pos, mas, table_mass, nfiles):
+    f = oracle.new_field(d)
+    if pos is not None and pos.shape[0]:
+        if table_mass == 0:
+            oracle.fieldize(box, d, f, pos, mas, 0.0, 1)
+        else:
+            oracle.fieldize(box, d, f, pos, None, table_mass, 1)
+    n = 0 if pos is None else pos.shape[0]
+    return f, h5u.expected_total_mass(mas, table_mass, n, nfiles)
+
+
+def _hdf5_particles(seed, shift=0.0):
+    rng = np.random.default_rng(seed)
+    box = 50.0
+    g = (np.arange(16) + 0.5) * (box / 16)
+    lat = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pos = [(rng.random((2500, 3)) * box).astype(np.float32),
+           np.mod(lat + rng.normal(0, 0.8, lat.shape) + shift, box).astype(np.float32), None, None,
+           (rng.random((300, 3)) * box).astype(np.float32), None]
+    mas = [(rng.random(2500) * 0.2 + 0.1).astype(np.float32), None, None, None,
+           (rng.random(300) * 0.3 + 0.05).astype(np.float32), None]
+    return box, pos, mas, [0.0, 0.75, 0.0, 0.0, 0.0, 0.0]
+
+
+hdf5_only = pytest.mark.skipif(not h5u.available(), reason="built without the HDF5 library")
+
+
+@hdf5_only
+@pytest.mark.parametrize("nfiles", [3, 0])
+def test_cli_hdf5_auto_and_stars(gpu, tmp_path, nfiles):
+    announce("cli_hdf5_auto", f"nfiles={nfiles}")
+    box, pos, mas, table = _hdf5_particles(21)
+    stem = str(tmp_path / "snap")
+    h5u.write_set(stem, pos, mas, table, box, nfiles)
+    d = grid_dims([0 if p is None else p.shape[0] for p in pos])
+    out = tmp_path / "out"
+    out.mkdir()
+    r = run(["-i", stem, "-o", str(out)])
+    assert f"FFT grid dimension: {d}" in r.stdout
+    nf = max(nfiles, 1)
+    assert f"Found {nf} hdf5 files in snapshot" in r.stderr
+    for t, name in [(0, "by"), (1, "DM"), (4, "st")]:
+        f, M = _hdf5_field(box, d, pos[t], mas[t], table[t], nfiles)
+        compare(out / f"PK-{name}-snap", expected(box, d, [(f, M)]))
+    assert sorted(os.listdir(out)) == ["PK-DM-snap", "PK-by-snap", "PK-st-snap"]
+    # -s 1: the stars join the baryons; the masses add, with no +1
+    out2 = tmp_path / "out2"
+    out2.mkdir()
+    run(["-i", stem, "-o", str(out2), "-s", "1"])
+    f0, M0 = _hdf5_field(box, d, pos[0], mas[0], table[0], nfiles)
+    f4, M4 = _hdf5_field(box, d, pos[4], mas[4], table[4], nfiles)
+    compare(out2 / "PK-by-snap", expected(box, d, [(f0 + f4, M0 + M4)]))
+
+
+@hdf5_only
+def test_cli_hdf5_cross_modes(gpu, tmp_path):
+    announce("cli_hdf5_cross")
+    box, pos, mas, table = _hdf5_particles(22)
+    _, pos2, mas2, table2 = _hdf5_particles(23, shift=0.4)
+    stem, other = str(tmp_path / "snap"), str(tmp_path / "other")
+    h5u.write_set(stem, pos, mas, table, box, 2)
+    h5u.write_set(other, pos2, mas2, table2, box, 3)
+    d = grid_dims([0 if p is None else p.shape[0] for p in pos])
+    # -c 4: dark matter x stars of one set
+    out = tmp_path / "c"
+    out.mkdir()
+    run(["-i", stem, "-o", str(out), "-c", "4"])
+    a = _hdf5_field(box, d, pos[1], mas[1], table[1], 2)
+    b = _hdf5_field(box, d, pos[4], mas[4], table[4], 2)
+    compare(out / "PK-DMxst-snap", expected(box, d, [a, b], cross=True))
+    # -j with an HDF5 second set: type 1 of both sets, under every type's name
+    out = tmp_path / "j"
+    out.mkdir()
+    run(["-i", stem, "-j", other, "-o", str(out)])
+    a = _hdf5_field(box, d, pos[1], mas[1], table[1], 2)
+    b = _hdf5_field(box, d, pos2[1], mas2[1], table2[1], 3)
+    exp = expected(box, d, [a, b], cross=True)
+    for name in ("by", "DM", "st"):
+        compare(out / f"PX-{name}-snap", exp)
+    # a set with a header that cannot be read: status 1
+    h5u.write_set(str(tmp_path / "bad"), pos, mas, table, box, 0)
+    with open(str(tmp_path / "bad.hdf5"), "r+b") as fh:
+        fh.seek(0)
+        fh.write(b"\0" * 8)  # no longer an HDF5 file: read as a (missing) Gadget set
+    r = run(["-i", str(tmp_path / "bad"), "-o", str(out)], ok=False)
+    assert r.returncode == 1
