@@ -146,13 +146,14 @@ template <typename T, int NM, int GM>
 __global__ void __launch_bounds__(GATHER_THREADS) k_gather_tiles(const T *__restrict__ G, const float *__restrict__ rec,
                                                                  const uint32_t *__restrict__ tile_start,
                                                                  const uint32_t *__restrict__ item_start,
+                                                                 const uint32_t *__restrict__ item_tile,
                                                                  GatherParams P, float *__restrict__ out) {
   constexpr int RW = 3 + NM;
   constexpr int RWO = 3 + GM;
   extern __shared__ unsigned char lds_raw[];
   T *g = reinterpret_cast<T *>(lds_raw);
   ItemRange ir;
-  if (!item_range(item_start, tile_start, P.ks, blockIdx.x, ir)) return;
+  if (!item_range(item_tile, item_start, tile_start, P.ks, blockIdx.x, ir)) return;
   if (ir.pb >= ir.pe) return;
   const int64_t k = ir.k;
   const int ntz = P.kg.ntz, nty = P.kg.nty, d = P.kg.d;
@@ -268,7 +269,7 @@ void gather_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass, fl
   P.fhy = FastDiv((uint32_t)tg.HY);
   hipLaunchKernelGGL((k_gather_tiles<T, NM, GM>), dim3((unsigned)o.max_items), dim3(GATHER_THREADS),
                      (size_t)tg.cells * sizeof(T), p.stream, p.field.as<const T>(), o.rec, o.tile_start, o.item_start,
-                     P, out);
+                     o.item_tile, P, out);
   GENPK_LAUNCH_CHECK();
 }
 

@@ -29,10 +29,12 @@ namespace {
 constexpr int DEP_THREADS = 1024;
 constexpr int FOLD_THREADS = 256;
 
-// 2^36 * w rounded to nearest, for w in [0, 1]: the integer lands in the low
-// mantissa bits of w * 2^36 + 2^52.
-__device__ __forceinline__ unsigned long long to_fixed(double w) {
-  const double t = fma(w, FIXED_ONE, 4503599627370496.0);
+// 2^36 * (a * b) rounded to nearest once, for a in [0, 1] and b = (a weight
+// in [0, 1]) * 2^36: the integer lands in the low mantissa bits of
+// a * b + 2^52.  bias = 2^52 comes from the kernel arguments, so it stays in
+// a scalar register and the product is one FMA.
+__device__ __forceinline__ unsigned long long to_fixed(double a, double b, double bias) {
+  const double t = fma(a, b, bias);
   return (unsigned long long)__double_as_longlong(t) - 0x4330000000000000ull;
 }
 
@@ -43,6 +45,7 @@ struct DepParams {
   int64_t cm, cz, zoff;  // tile store layout
   int64_t ks;          // key space size (tile_start has ks+1 entries)
   double mass;         // constant mass (u64 path)
+  double bias;         // 2^52 (to_fixed)
   int store;           // 1: write tiles, 0: add to tiles
 };
 
@@ -54,6 +57,7 @@ template <typename Real, int NM, int RW, bool STD>
 __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict__ rec,
                                                          const uint32_t *__restrict__ tile_start,
                                                          const uint32_t *__restrict__ item_start,
+                                                         const uint32_t *__restrict__ item_tile,
                                                          const uint32_t *__restrict__ part_start, DepParams P,
                                                          Real *__restrict__ tiles, double *__restrict__ partials) {
   using Acc = typename std::conditional<NM == 1, double, unsigned long long>::type;
@@ -61,7 +65,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
   extern __shared__ unsigned long long lds_raw[];
   Acc *acc = reinterpret_cast<Acc *>(lds_raw);
   ItemRange ir;
-  if (!item_range(item_start, tile_start, P.ks, blockIdx.x, ir)) return;
+  if (!item_range(item_tile, item_start, tile_start, P.ks, blockIdx.x, ir)) return;
   const int64_t k = ir.k;
   const uint32_t m = ir.m, j = ir.j, pb = ir.pb, pe = ir.pe;
 
@@ -110,6 +114,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
         continue;  // cannot happen for records of this tile; guards the LDS bounds
       const double gx = 1.0 - fx, gy = 1.0 - fy, gz = 1.0 - fz;
       const double wxy[4] = {gx * gy, fx * gy, gx * fy, fx * fy};
+      const double gzs = gz * FIXED_ONE, fzs = fz * FIXED_ONE;  // exact: a power-of-two scale
       const int base = lx * sx + ly * sy + lz;
       const int off[4] = {0, sx, sy, sx + sy};
 #pragma unroll
@@ -120,8 +125,8 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
           atomicAdd(&acc[at], mw * gz);
           atomicAdd(&acc[at + 1], mw * fz);
         } else {
-          atomicAdd(&acc[at], to_fixed(wxy[c] * gz));
-          atomicAdd(&acc[at + 1], to_fixed(wxy[c] * fz));
+          atomicAdd(&acc[at], to_fixed(wxy[c], gzs, P.bias));
+          atomicAdd(&acc[at + 1], to_fixed(wxy[c], fzs, P.bias));
         }
       }
     }
@@ -255,6 +260,7 @@ void deposit_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass) {
   P.zoff = tg.K * tg.cm;
   P.ks = ks;
   P.mass = mass;
+  P.bias = 4503599627370496.0;
   P.store = store;
   const size_t lds = (size_t)tg.cells * 8;
   {
@@ -263,16 +269,16 @@ void deposit_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass) {
     const dim3 grid((unsigned)max_items), block(DEP_THREADS);
     if (p.prec == 4 && std_tiles)
       hipLaunchKernelGGL((k_deposit<float, NM, RW, true>), grid, block, lds, s, sorted, tile_start, item_start,
-                         part_start, P, p.tiles.as<float>(), partials);
+                         o.item_tile, part_start, P, p.tiles.as<float>(), partials);
     else if (p.prec == 4)
       hipLaunchKernelGGL((k_deposit<float, NM, RW, false>), grid, block, lds, s, sorted, tile_start, item_start,
-                         part_start, P, p.tiles.as<float>(), partials);
+                         o.item_tile, part_start, P, p.tiles.as<float>(), partials);
     else if (std_tiles)
       hipLaunchKernelGGL((k_deposit<double, NM, RW, true>), grid, block, lds, s, sorted, tile_start, item_start,
-                         part_start, P, p.tiles.as<double>(), partials);
+                         o.item_tile, part_start, P, p.tiles.as<double>(), partials);
     else
       hipLaunchKernelGGL((k_deposit<double, NM, RW, false>), grid, block, lds, s, sorted, tile_start, item_start,
-                         part_start, P, p.tiles.as<double>(), partials);
+                         o.item_tile, part_start, P, p.tiles.as<double>(), partials);
     GENPK_LAUNCH_CHECK();
   }
   {

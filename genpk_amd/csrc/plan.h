@@ -92,6 +92,7 @@ struct TileOrder {
   bool has_mass = false, has_perm = false;
   const uint32_t *perm1 = nullptr, *perm2 = nullptr;
   const uint32_t *tile_start = nullptr, *item_start = nullptr, *part_start = nullptr;
+  const uint32_t *item_tile = nullptr;  // tile of each work item (max_items entries)
   const uint32_t *nheavy = nullptr, *heavy = nullptr;
   int64_t ks = 0, n = 0, max_items = 0;
 };
@@ -148,7 +149,7 @@ struct Plan {
   bool tiles_live = false;  // tiles hold deposits not yet folded
 
   // sort / deposit scratch
-  DevBuf recA, recB, table1, table2, seginfo, tile_start, item_start, part_start, heavy, partials;
+  DevBuf recA, recB, table1, table2, seginfo, tile_start, item_start, part_start, heavy, partials, item_tile;
   DevBuf permA, permB;      // sort permutations kept for the adjoint gather
   DevBuf scan_tmp;
   RetainedOrder order;

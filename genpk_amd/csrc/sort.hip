@@ -411,6 +411,13 @@ __global__ void k_items(const uint32_t *__restrict__ tile_start, int64_t ks, int
   if (k == 0) *nheavy = 0;
 }
 
+// item_tile[i] = tile of work item i
+__global__ void k_item_tiles(const uint32_t *__restrict__ item_start, int64_t ks, uint32_t *__restrict__ item_tile) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ks) return;
+  for (uint32_t i = item_start[k]; i < item_start[k + 1]; i++) item_tile[i] = (uint32_t)k;
+}
+
 __global__ void k_heavy_list(const uint32_t *__restrict__ pcnt, int64_t ks, uint32_t *__restrict__ nheavy,
                              uint32_t *__restrict__ heavy) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -564,8 +571,11 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
   exclusive_scan_u32(pcnt, part_start, ks + 1, p.scan_tmp, s);
   hipLaunchKernelGGL(k_heavy_list, dim3((unsigned)ceil_div(ks, 256)), dim3(256), 0, s, pcnt, ks, nheavy, heavy);
   GENPK_LAUNCH_CHECK();
-
   const int64_t max_items = tg.K + ceil_div(n, DEPOSIT_ITEM_MAX);
+  uint32_t *item_tile = static_cast<uint32_t *>(p.item_tile.ensure((size_t)max_items * 4));
+  hipLaunchKernelGGL(k_item_tiles, dim3((unsigned)ceil_div(ks, 256)), dim3(256), 0, s, item_start, ks, item_tile);
+  GENPK_LAUNCH_CHECK();
+
   TileOrder o;
   o.rec = sorted;
   o.rw = Src<NM>::RW;
@@ -575,6 +585,7 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
   o.perm2 = perm2;
   o.tile_start = tile_start;
   o.item_start = item_start;
+  o.item_tile = item_tile;
   o.part_start = part_start;
   o.nheavy = nheavy;
   o.heavy = heavy;

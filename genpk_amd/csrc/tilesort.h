@@ -41,24 +41,25 @@ __device__ __forceinline__ uint32_t tile_key(const float *r, const KeyGeom &g) {
 
 // Work item -> tile k and its particle range [pb, pe): tile k has m items
 // (item_start[k+1] - item_start[k]) that split its records evenly; j is the
-// item's number within the tile.  Returns false past the last item.
+// item's number within the tile.  item_tile[item] is the item's tile (read
+// for every item < max_items, used only below the item count).  Returns
+// false past the last item.
 struct ItemRange {
   int64_t k;
   uint32_t m, j, pb, pe;
 };
-__device__ __forceinline__ bool item_range(const uint32_t *__restrict__ item_start,
+__device__ __forceinline__ bool item_range(const uint32_t *__restrict__ item_tile,
+                                           const uint32_t *__restrict__ item_start,
                                            const uint32_t *__restrict__ tile_start, int64_t ks, uint32_t item,
                                            ItemRange &r) {
-  if (item >= item_start[ks]) return false;
-  int64_t lo = 0, hi = ks;
-  while (hi - lo > 1) {
-    const int64_t mid = (lo + hi) >> 1;
-    if (item_start[mid] <= item) lo = mid; else hi = mid;
-  }
-  r.k = lo;
-  r.m = item_start[lo + 1] - item_start[lo];
-  r.j = item - item_start[lo];
-  const uint32_t t0 = tile_start[lo], t1 = tile_start[lo + 1];
+  const uint32_t total = item_start[ks];
+  const uint32_t k = item_tile[item];
+  if (item >= total) return false;
+  r.k = k;
+  const uint32_t i0 = item_start[k], i1 = item_start[k + 1];
+  const uint32_t t0 = tile_start[k], t1 = tile_start[k + 1];
+  r.m = i1 - i0;
+  r.j = item - i0;
   const uint32_t per = (t1 - t0 + r.m - 1) / r.m;
   r.pb = min(t1, t0 + r.j * per);
   r.pe = min(t1, r.pb + per);
