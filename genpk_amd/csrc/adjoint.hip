@@ -133,6 +133,7 @@ struct GatherParams {
   int HY, HZ;
   int64_t cells;
   int64_t ks;
+  int64_t max_items;  // entries of the item-to-tile table
   int64_t rowlen;
   double mass;
   FastDiv fhz, fhy;
@@ -153,7 +154,7 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather_tiles(const T *__rest
   extern __shared__ unsigned char lds_raw[];
   T *g = reinterpret_cast<T *>(lds_raw);
   ItemRange ir;
-  if (!item_range(item_tile, item_start, tile_start, P.ks, blockIdx.x, ir)) return;
+  if (!item_range(item_tile, item_start, tile_start, P.ks, P.max_items, blockIdx.x, ir)) return;
   if (ir.pb >= ir.pe) return;
   const int64_t k = ir.k;
   const int ntz = P.kg.ntz, nty = P.kg.nty, d = P.kg.d;
@@ -263,6 +264,7 @@ void gather_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass, fl
   P.HZ = tg.HZ;
   P.cells = tg.cells;
   P.ks = o.ks;
+  P.max_items = o.max_items;
   P.rowlen = p.rowlen;
   P.mass = mass;
   P.fhz = FastDiv((uint32_t)tg.HZ);

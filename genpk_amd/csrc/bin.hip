@@ -352,6 +352,8 @@ __global__ void k_bin_finish(int nrbins, int ns, int G, const double *__restrict
   if (keffs_out) keffs_out[b] = c ? keffsum[b] / c : 0.0;
 }
 
+}  // namespace
+
 int num_cus() {
   static int cus = 0;
   if (!cus) {
@@ -361,8 +363,6 @@ int num_cus() {
   }
   return cus;
 }
-
-}  // namespace
 
 // float(onedinvwindow(KVAL(k), d)) for k < d, the factor of fieldize.cpp:117-121
 static double onedinvwindow_host(int64_t kx, int64_t n) {

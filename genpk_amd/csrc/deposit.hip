@@ -44,6 +44,7 @@ struct DepParams {
   int64_t cells;       // accumulator cells HX*HY*HZ
   int64_t cm, cz, zoff;  // tile store layout
   int64_t ks;          // key space size (tile_start has ks+1 entries)
+  int64_t max_items;   // entries of the item-to-tile table
   double mass;         // constant mass (u64 path)
   double bias;         // 2^52 (to_fixed)
   int store;           // 1: write tiles, 0: add to tiles
@@ -62,99 +63,107 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
                                                          Real *__restrict__ tiles, double *__restrict__ partials) {
   using Acc = typename std::conditional<NM == 1, double, unsigned long long>::type;
   constexpr int UNR = 4;
+  constexpr int BATCH = DEP_THREADS * UNR;
   extern __shared__ unsigned long long lds_raw[];
   Acc *acc = reinterpret_cast<Acc *>(lds_raw);
-  ItemRange ir;
-  if (!item_range(item_tile, item_start, tile_start, P.ks, blockIdx.x, ir)) return;
-  const int64_t k = ir.k;
-  const uint32_t m = ir.m, j = ir.j, pb = ir.pb, pe = ir.pe;
-
-  for (int64_t c = threadIdx.x; c < P.cells; c += DEP_THREADS) acc[c] = 0;
-  __syncthreads();
-
-  const int ntz = P.kg.ntz, nty = P.kg.nty;
-  const int tz = (int)(k % ntz);
-  const int ty = (int)((k / ntz) % nty);
-  const int tx = (int)(k / ((int64_t)ntz * nty));
   const int TX = STD ? 16 : P.kg.TX, TY = STD ? 32 : P.kg.TY, TZ = STD ? 32 : P.kg.TZ;
-  const int ox = tx * TX, oy = ty * TY, oz = tz * TZ;
   const int sy = STD ? 33 : P.HZ, sx = STD ? 33 * 33 : P.HY * P.HZ;
+  const double scale = (NM == 1) ? 1.0 : P.mass * FIXED_INV;
   float r[UNR][RW], nx[UNR][RW];
-#pragma unroll
-  for (int u = 0; u < UNR; u++) {
-    const uint32_t i = pb + u * DEP_THREADS + threadIdx.x;
-    if (i < pe) {
-#pragma unroll
-      for (int q = 0; q < RW; q++) nx[u][q] = rec[(int64_t)i * RW + q];
-    }
-  }
-  for (uint32_t i0 = pb; i0 < pe; i0 += DEP_THREADS * UNR) {
-#pragma unroll
-    for (int u = 0; u < UNR; u++)
-#pragma unroll
-      for (int q = 0; q < RW; q++) r[u][q] = nx[u][q];
-    // loads of the next batch are issued before this batch is accumulated
+  auto load_batch = [&](uint32_t i0, uint32_t pe) {
 #pragma unroll
     for (int u = 0; u < UNR; u++) {
-      const uint32_t i = i0 + (UNR + u) * DEP_THREADS + threadIdx.x;
+      const uint32_t i = i0 + u * DEP_THREADS + threadIdx.x;
       if (i < pe) {
 #pragma unroll
         for (int q = 0; q < RW; q++) nx[u][q] = rec[(int64_t)i * RW + q];
       }
     }
+  };
+  // Persistent over the items (item, item + gridDim.x, ...): the next item's
+  // tile lookup and first records load while this item's tile is flushed.
+  uint32_t item = blockIdx.x;
+  ItemRange ir;
+  bool have = item_range(item_tile, item_start, tile_start, P.ks, P.max_items, item, ir);
+  if (have) load_batch(ir.pb, ir.pe);
+  while (have) {
+    const int64_t k = ir.k;
+    const uint32_t m = ir.m, j = ir.j, pb = ir.pb, pe = ir.pe;
+    for (int64_t c = threadIdx.x; c < P.cells; c += DEP_THREADS) acc[c] = 0;
+    __syncthreads();
+    const int ntz = P.kg.ntz, nty = P.kg.nty;
+    const int tz = (int)(k % ntz);
+    const int ty = (int)((k / ntz) % nty);
+    const int tx = (int)(k / ((int64_t)ntz * nty));
+    const int ox = tx * TX, oy = ty * TY, oz = tz * TZ;
+    for (uint32_t i0 = pb; i0 < pe; i0 += BATCH) {
 #pragma unroll
-    for (int u = 0; u < UNR; u++) {
-      const uint32_t i = i0 + u * DEP_THREADS + threadIdx.x;
-      if (i >= pe) continue;
-      double fx, fy, fz;
-      const int lx = cell_of(r[u][0], P.kg.units, P.kg.d, fx) - ox;
-      const int ly = cell_of(r[u][1], P.kg.units, P.kg.d, fy) - oy;
-      const int lz = cell_of(r[u][2], P.kg.units, P.kg.d, fz) - oz;
-      if ((unsigned)lx >= (unsigned)TX || (unsigned)ly >= (unsigned)TY || (unsigned)lz >= (unsigned)TZ)
-        continue;  // cannot happen for records of this tile; guards the LDS bounds
-      const double gx = 1.0 - fx, gy = 1.0 - fy, gz = 1.0 - fz;
-      const double wxy[4] = {gx * gy, fx * gy, gx * fy, fx * fy};
-      const double gzs = gz * FIXED_ONE, fzs = fz * FIXED_ONE;  // exact: a power-of-two scale
-      const int base = lx * sx + ly * sy + lz;
-      const int off[4] = {0, sx, sy, sx + sy};
+      for (int u = 0; u < UNR; u++)
 #pragma unroll
-      for (int c = 0; c < 4; c++) {
-        const int at = base + off[c];
-        if constexpr (NM == 1) {
-          const double mw = (double)r[u][3] * wxy[c];
-          atomicAdd(&acc[at], mw * gz);
-          atomicAdd(&acc[at + 1], mw * fz);
-        } else {
-          atomicAdd(&acc[at], to_fixed(wxy[c], gzs, P.bias));
-          atomicAdd(&acc[at + 1], to_fixed(wxy[c], fzs, P.bias));
+        for (int q = 0; q < RW; q++) r[u][q] = nx[u][q];
+      // loads of the next batch are issued before this batch is accumulated
+      if (i0 + BATCH < pe) load_batch(i0 + BATCH, pe);
+#pragma unroll
+      for (int u = 0; u < UNR; u++) {
+        const uint32_t i = i0 + u * DEP_THREADS + threadIdx.x;
+        if (i >= pe) continue;
+        double fx, fy, fz;
+        const int lx = cell_of(r[u][0], P.kg.units, P.kg.d, fx) - ox;
+        const int ly = cell_of(r[u][1], P.kg.units, P.kg.d, fy) - oy;
+        const int lz = cell_of(r[u][2], P.kg.units, P.kg.d, fz) - oz;
+        if ((unsigned)lx >= (unsigned)TX || (unsigned)ly >= (unsigned)TY || (unsigned)lz >= (unsigned)TZ)
+          continue;  // cannot happen for records of this tile; guards the LDS bounds
+        const double gx = 1.0 - fx, gy = 1.0 - fy, gz = 1.0 - fz;
+        const double wxy[4] = {gx * gy, fx * gy, gx * fy, fx * fy};
+        const double gzs = gz * FIXED_ONE, fzs = fz * FIXED_ONE;  // exact: a power-of-two scale
+        const int base = lx * sx + ly * sy + lz;
+        const int off[4] = {0, sx, sy, sx + sy};
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const int at = base + off[c];
+          if constexpr (NM == 1) {
+            const double mw = (double)r[u][3] * wxy[c];
+            atomicAdd(&acc[at], mw * gz);
+            atomicAdd(&acc[at + 1], mw * fz);
+          } else {
+            atomicAdd(&acc[at], to_fixed(wxy[c], gzs, P.bias));
+            atomicAdd(&acc[at + 1], to_fixed(wxy[c], fzs, P.bias));
+          }
         }
       }
     }
-  }
-  __syncthreads();
-  const double scale = (NM == 1) ? 1.0 : P.mass * FIXED_INV;
-  if (m == 1) {
-    // rows of TZ values (aligned), then the z-halo plane
-    Real *rows = tiles + k * P.cm;
-    Real *zh = tiles + P.zoff + k * P.cz;
-    const int sz = P.kg.sz, TZ = P.kg.TZ;
-    if (P.store) {
-      for (int64_t c = threadIdx.x; c < P.cm; c += DEP_THREADS) {
-        const int64_t a = (c >> sz) * P.HZ + (c & (TZ - 1));
-        rows[c] = (Real)((double)acc[a] * scale);
+    __syncthreads();
+    // the next item (uniform over the workgroup); its first records load during the flush
+    item += gridDim.x;
+    ItemRange nir;
+    const bool next = item_range(item_tile, item_start, tile_start, P.ks, P.max_items, item, nir);
+    if (next) load_batch(nir.pb, nir.pe);
+    if (m == 1) {
+      // rows of TZ values (aligned), then the z-halo plane
+      Real *rows = tiles + k * P.cm;
+      Real *zh = tiles + P.zoff + k * P.cz;
+      const int sz = P.kg.sz;
+      if (P.store) {
+        for (int64_t c = threadIdx.x; c < P.cm; c += DEP_THREADS) {
+          const int64_t a2 = (c >> sz) * P.HZ + (c & (TZ - 1));
+          rows[c] = (Real)((double)acc[a2] * scale);
+        }
+        for (int64_t c = threadIdx.x; c < P.cz; c += DEP_THREADS) zh[c] = (Real)((double)acc[c * P.HZ + TZ] * scale);
+      } else {
+        for (int64_t c = threadIdx.x; c < P.cm; c += DEP_THREADS) {
+          const int64_t a2 = (c >> sz) * P.HZ + (c & (TZ - 1));
+          rows[c] = (Real)((double)rows[c] + (double)acc[a2] * scale);
+        }
+        for (int64_t c = threadIdx.x; c < P.cz; c += DEP_THREADS)
+          zh[c] = (Real)((double)zh[c] + (double)acc[c * P.HZ + TZ] * scale);
       }
-      for (int64_t c = threadIdx.x; c < P.cz; c += DEP_THREADS) zh[c] = (Real)((double)acc[c * P.HZ + TZ] * scale);
     } else {
-      for (int64_t c = threadIdx.x; c < P.cm; c += DEP_THREADS) {
-        const int64_t a = (c >> sz) * P.HZ + (c & (TZ - 1));
-        rows[c] = (Real)((double)rows[c] + (double)acc[a] * scale);
-      }
-      for (int64_t c = threadIdx.x; c < P.cz; c += DEP_THREADS)
-        zh[c] = (Real)((double)zh[c] + (double)acc[c * P.HZ + TZ] * scale);
+      double *out = partials + (int64_t)(part_start[k] + j) * P.cells;
+      for (int64_t c = threadIdx.x; c < P.cells; c += DEP_THREADS) out[c] = (double)acc[c] * scale;
     }
-  } else {
-    double *out = partials + (int64_t)(part_start[k] + j) * P.cells;
-    for (int64_t c = threadIdx.x; c < P.cells; c += DEP_THREADS) out[c] = (double)acc[c] * scale;
+    __syncthreads();  // the flush has read the tile before it is cleared again
+    ir = nir;
+    have = next;
   }
 }
 
@@ -239,6 +248,25 @@ __global__ void __launch_bounds__(FOLD_THREADS) k_fold(const Real *__restrict__ 
   *o = accumulate ? (Out)((double)*o + v) : (Out)v;
 }
 
+// Persistent grid of the deposit: the workgroups that stay resident (one
+// per CU with a 147 KB tile), at most one per item.
+template <typename Real, int NM, int RW, bool STD>
+int64_t deposit_grid_of(size_t lds, int64_t max_items) {
+  int per_cu = 0;
+  GENPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void *>(&k_deposit<Real, NM, RW, STD>), DEP_THREADS, lds));
+  return std::max<int64_t>(1, std::min<int64_t>((int64_t)std::max(per_cu, 1) * num_cus(), max_items));
+}
+
+template <int NM, int RW>
+int64_t deposit_grid(const Plan &p, bool std_tiles, size_t lds, int64_t max_items) {
+  if (p.prec == 4)
+    return std_tiles ? deposit_grid_of<float, NM, RW, true>(lds, max_items)
+                     : deposit_grid_of<float, NM, RW, false>(lds, max_items);
+  return std_tiles ? deposit_grid_of<double, NM, RW, true>(lds, max_items)
+                   : deposit_grid_of<double, NM, RW, false>(lds, max_items);
+}
+
 template <int NM, int RW>
 void deposit_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass) {
   hipStream_t s = p.stream;
@@ -259,6 +287,7 @@ void deposit_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass) {
   P.cz = tg.cz;
   P.zoff = tg.K * tg.cm;
   P.ks = ks;
+  P.max_items = max_items;
   P.mass = mass;
   P.bias = 4503599627370496.0;
   P.store = store;
@@ -266,7 +295,7 @@ void deposit_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass) {
   {
     Stage st(p, "deposit_tiles");
     const bool std_tiles = tg.TX == 16 && tg.TY == 32 && tg.TZ == 32;
-    const dim3 grid((unsigned)max_items), block(DEP_THREADS);
+    const dim3 grid((unsigned)deposit_grid<NM, RW>(p, std_tiles, lds, max_items)), block(DEP_THREADS);
     if (p.prec == 4 && std_tiles)
       hipLaunchKernelGGL((k_deposit<float, NM, RW, true>), grid, block, lds, s, sorted, tile_start, item_start,
                          o.item_tile, part_start, P, p.tiles.as<float>(), partials);

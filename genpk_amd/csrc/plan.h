@@ -193,6 +193,8 @@ bool plan_fft_ok(const Plan &p);
 // deposit.hip
 void deposit(Plan &p, double boxsize, int64_t n, const float *pos, const float *masses, double mass,
              bool retain = false);
+// Compute units of the current device.
+int num_cus();
 void fold_tiles(Plan &p, void *out, int64_t out_rowlen, bool accumulate);
 void fill_zero(void *ptr, size_t bytes, hipStream_t s);
 void tiles_to_field(Plan &p);

@@ -41,8 +41,9 @@ __device__ __forceinline__ uint32_t tile_key(const float *r, const KeyGeom &g) {
 
 // Work item -> tile k and its particle range [pb, pe): tile k has m items
 // (item_start[k+1] - item_start[k]) that split its records evenly; j is the
-// item's number within the tile.  item_tile[item] is the item's tile (read
-// for every item < max_items, used only below the item count).  Returns
+// item's number within the tile.  item_tile (ntab entries, at least the
+// item count) holds each item's tile; it is read at min(item, ntab - 1)
+// before the item count is known, and used only below the count.  Returns
 // false past the last item.
 struct ItemRange {
   int64_t k;
@@ -50,10 +51,10 @@ struct ItemRange {
 };
 __device__ __forceinline__ bool item_range(const uint32_t *__restrict__ item_tile,
                                            const uint32_t *__restrict__ item_start,
-                                           const uint32_t *__restrict__ tile_start, int64_t ks, uint32_t item,
-                                           ItemRange &r) {
+                                           const uint32_t *__restrict__ tile_start, int64_t ks, int64_t ntab,
+                                           uint32_t item, ItemRange &r) {
   const uint32_t total = item_start[ks];
-  const uint32_t k = item_tile[item];
+  const uint32_t k = item_tile[min((int64_t)item, ntab - 1)];
   if (item >= total) return false;
   r.k = k;
   const uint32_t i0 = item_start[k], i1 = item_start[k + 1];
