@@ -172,8 +172,9 @@ __global__ void __launch_bounds__(SORT_THREADS) k_upsweep(Src<NM> src, ChunkMap 
 // in this sub-chunk, from LDS atomic adds that return the old count on the
 // wave's count row (a wave whose records share one digit adds them with one
 // atomic).  Equal digits are ordered by the LDS unit's processing order of
-// one instruction's lanes, so the partition is not guaranteed to be stable;
-// per-wave count rows keep the result independent of wave scheduling.
+// one instruction's lanes and, where waves share a count row, by the order
+// in which they run: the order of the records within a tile is not
+// specified (the tile kernels do not depend on it).
 template <int NM, int PL>
 __device__ __forceinline__ void rank_records(const float (&rec)[PL][Src<NM>::RW], int wbase, int nsub,
                                              const KeyGeom &kg, const DigitSel &ds, uint32_t *wrow,
@@ -254,8 +255,7 @@ __device__ __forceinline__ uint32_t block_scan_round(uint32_t v, uint32_t (*wtot
 // the per-wave counts are double-buffered: this sub-chunk's buffer is
 // cleared during the write-out and the next sub-chunk ranks into the other
 // one, so no barrier separates them.  hw count rows (a divisor of the wave
-// count) serve the waves, wave w on row w % hw: with fewer rows than waves
-// (wide digits) the order of equal digits also depends on wave scheduling.
+// count) serve the waves, wave w on row w % hw.
 template <int NM, int PL, bool DB>
 __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, float *__restrict__ dst, ChunkMap cm,
                                                                KeyGeom kg, DigitSel ds, int nb, int hw,
@@ -436,11 +436,13 @@ size_t downsweep_lds(int nb, int rw, const DsShape &h) {
   return (size_t)((h.db ? 2 : 1) * h.hw * nb + 3 * nb) * 4 + sub * 2 + sub * rw * 4;
 }
 
-// The first shape whose LDS fits one workgroup per CU: a count row per wave
-// when possible, then the most records per lane, then double-buffered counts.
+// The first shape whose LDS fits one workgroup per CU: four count rows (each
+// shared by four waves: 16 rows measured 0.1 ms slower per downsweep at 2^30
+// records, the per-digit offsets running over more rows), then the most
+// records per lane, then double-buffered counts.
 DsShape downsweep_shape(int nb, int rw) {
   const size_t cap = 160 * 1024 - 256;
-  for (int hw : {SORT_WAVES, SORT_WAVES / 4})
+  for (int hw : {SORT_WAVES / 4, 1})
     for (int pl : {8, 4, 2})
       for (bool db : {true, false}) {
         const DsShape h{pl, db, hw};
