@@ -8,16 +8,17 @@
 //   -s 1                                 add the stars (type 4) to the baryons (type 0)
 //   -p 32|64                             grid/FFT precision (default 64, the reference's double)
 //
-// Snapshots are Gadget format I/II file sets, or Gadget-HDF5 file sets when
-// built with the HDF5 library (GENPK_HDF5).  Particles are read on the host
+// Snapshots are Gadget format I/II file sets, MP-Gadget bigfile snapshots, or
+// Gadget-HDF5 file sets when built with the HDF5 library (GENPK_HDF5).  Particles are read on the host
 // (Gadget: chunks of at most 2^26, as the reference; HDF5: a file at a time),
 // copied to the device in chunks of at most 2^26 and deposited into a plan;
 // the fused transform + binning runs on the device.  The total mass of a
 // species is the sum of its particle masses (header mass x count, or the
 // MASS block / Masses dataset), plus 1 per read of a species from a Gadget
-// set (read_fieldize.cpp:94); HDF5 reads add nothing (:320-331).  Bigfile
-// snapshots are not supported.
+// set (read_fieldize.cpp:94); HDF5 and bigfile reads add nothing
+// (read_fieldize.cpp:320-331, read_fieldize_bigfile.cpp:94-114).
 #include "../../include/genpk_amd.h"
+#include "bigfile.h"
 #include "gadget.h"
 #ifdef GENPK_HDF5
 #include "hdf5snap.h"
@@ -172,20 +173,51 @@ void read_fieldize_hdf5(genpk_plan *plan, const std::vector<std::string> &files,
 }
 #endif
 
-// A snapshot: a Gadget set or an HDF5 set.
+// read_fieldize_bigfile (read_fieldize_bigfile.cpp:63-125): every particle
+// of `type`; the masses summed in double in block order, then + MassTable x
+// count.  A block that cannot be read ends the program, as the reference
+// cannot go on without its coordinates.
+void read_fieldize_bigfile(genpk_plan *plan, const std::string &name, const bigfile::Header &h, int type,
+                           double box, double *total_mass, Staging &st, hipStream_t s) {
+  std::vector<float> pos, masses;
+  std::string err;
+  if (!bigfile::read_type(name, type, h, pos, masses, err)) die("Failed to read from block: " + err);
+  const int64_t n = h.npart_total[type];
+  if (n == 0) return;
+  const bool per_particle = h.mass[type] == 0;
+  if (per_particle) {
+    double this_file = 0;
+    for (int64_t i = 0; i < n; i++) this_file += masses[(size_t)i];
+    *total_mass += this_file;
+  }
+  *total_mass += h.mass[type] * n;
+  deposit_host(plan, box, n, pos.data(), per_particle ? masses.data() : nullptr, h.mass[type], st, s);
+}
+
+// A snapshot: a Gadget set, an HDF5 set (found first, as the reference does)
+// or a bigfile snapshot.
 struct Source {
   std::unique_ptr<gadget::Snapshot> gadget;
   std::vector<std::string> hdf5;
+  std::string big;        // bigfile snapshot directory
+  bigfile::Header bighdr;
   explicit Source(const std::string &name) {
 #ifdef GENPK_HDF5
     hdf5 = hdf5snap::find_set(name);
     if (!hdf5.empty()) return;
 #endif
+    if (bigfile::is_bigfile(name)) {
+      std::string err;
+      if (!bigfile::load_header(name, bighdr, err)) die(err + "\nCould not load header");
+      big = name;
+      return;
+    }
     gadget = std::make_unique<gadget::Snapshot>(name);
   }
   bool is_hdf5() const { return !hdf5.empty(); }
+  bool is_bigfile() const { return !big.empty(); }
   // Gadget: read_fieldize, returning 1 when the snapshot holds none of the
-  // type; HDF5: read_fieldize_hdf5 over the files, returning 0.
+  // type; HDF5 and bigfile: their readers, returning 0.
   int read(genpk_plan *plan, int type, double box, double *total_mass, Staging &st, hipStream_t s) const {
 #ifdef GENPK_HDF5
     if (is_hdf5()) {
@@ -193,6 +225,10 @@ struct Source {
       return 0;
     }
 #endif
+    if (is_bigfile()) {
+      read_fieldize_bigfile(plan, big, bighdr, type, box, total_mass, st, s);
+      return 0;
+    }
     return read_fieldize(plan, *gadget, type, box, total_mass, st, s);
   }
 };
@@ -277,6 +313,17 @@ int main(int argc, char *argv[]) {
       std::printf("Expansion factor = %f\n", h.time);
       std::printf("Hubble = %g Box=%g \n", h.h100, h.box);
 #endif
+    } else if (snap.is_bigfile()) {
+      // load_bigfile_header's console lines (read_fieldize_bigfile.cpp:53-58)
+      const bigfile::Header &h = snap.bighdr;
+      for (int t = 0; t < gadget::N_TYPE; t++) npart_total[t] = h.npart_total[t];
+      box = h.box;
+      std::printf("NumPart=[%ld,%ld,%ld,%ld,%ld,%ld], ", (long)npart_total[0], (long)npart_total[1],
+                  (long)npart_total[2], (long)npart_total[3], (long)npart_total[4], (long)npart_total[5]);
+      std::printf("Masses=[%g %g %g %g %g %g], ", h.mass[0], h.mass[1], h.mass[2], h.mass[3], h.mass[4], h.mass[5]);
+      std::printf("Redshift=%g, \xce\xa9_M=%g\n", h.redshift, h.omega0);
+      std::printf("Expansion factor = %f\n", h.time);
+      std::printf("Hubble = %g Box=%g \n", h.h100, h.box);
     } else {
       const gadget::Header &h = snap.gadget->header();
       for (int t = 0; t < gadget::N_TYPE; t++) npart_total[t] = snap.gadget->npart(t);
@@ -326,8 +373,10 @@ int main(int argc, char *argv[]) {
       // With an HDF5 second set the reference crosses type 1 of the two sets
       // under every type's name (gen-pk.cpp:271-288); so does this driver
       // (the reference reads nothing for the first set when that one is a
-      // Gadget set).  An HDF5 first set with a Gadget second set reads each
-      // type from both (the reference dereferences a missing snapshot there).
+      // Gadget set).  Otherwise each type is read from both sets, whatever
+      // their formats (with an HDF5 first set and a Gadget second set the
+      // reference dereferences a missing snapshot; with a bigfile first set
+      // it reads the first set twice into the first field).
       const bool dm_only = snap2.is_hdf5();
       ok(genpk_plan_create(&b, field_dims, precision, s), "plan");
       for (int type = 0; type < gadget::N_TYPE; type++) {

@@ -282,3 +282,63 @@ def test_cli_hdf5_cross_modes(gpu, tmp_path):
         fh.write(b"\0" * 8)  # no longer an HDF5 file: read as a (missing) Gadget set
     r = run(["-i", str(tmp_path / "bad"), "-o", str(out)], ok=False)
     assert r.returncode == 1
+
+
+# ---- MP-Gadget bigfile snapshots (read_fieldize_bigfile.cpp:7-125; gen-pk.cpp:135-145,
+# 217-221, 320-330): positions read as float64 and rounded to float32, no +1.
+
+import bigfile_util as bfu  # noqa: E402
+
+
+def _bigfile_particles(seed, shift=0.0):
+    rng = np.random.default_rng(seed)
+    box = 50.0
+    g = (np.arange(16) + 0.5) * (box / 16)
+    lat = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pos = [rng.random((2500, 3)) * box, np.mod(lat + rng.normal(0, 0.8, lat.shape) + shift, box), None, None,
+           rng.random((300, 3)) * box, None]
+    mas = [(rng.random(2500) * 0.2 + 0.1).astype(np.float32), None, None, None,
+           (rng.random(300) * 0.3 + 0.05).astype(np.float32), None]
+    return box, pos, mas, [0.0, 0.75, 0.0, 0.0, 0.0, 0.0]
+
+
+def _bigfile_field(box, d, pos, mas, table_mass):
+    f = oracle.new_field(d)
+    p32 = np.asarray(pos, dtype=np.float64).astype(np.float32)
+    if table_mass == 0:
+        oracle.fieldize(box, d, f, p32, mas, 0.0, 1)
+    else:
+        oracle.fieldize(box, d, f, p32, None, table_mass, 1)
+    return f, bfu.expected_total_mass(mas, table_mass, p32.shape[0])
+
+
+def test_cli_bigfile(gpu, tmp_path):
+    announce("cli_bigfile")
+    box, pos, mas, table = _bigfile_particles(31)
+    snap = str(tmp_path / "PART_010")
+    bfu.write_snapshot(snap, pos, mas, table, box, nfiles=3)
+    d = grid_dims([0 if p is None else len(p) for p in pos])
+    out = tmp_path / "out"
+    out.mkdir()
+    r = run(["-i", snap, "-o", str(out)])
+    assert f"FFT grid dimension: {d}" in r.stdout and "Expansion factor" in r.stdout
+    for t, name in [(0, "by"), (1, "DM"), (4, "st")]:
+        compare(out / f"PK-{name}-PART_010", expected(box, d, [_bigfile_field(box, d, pos[t], mas[t], table[t])]))
+    # -c 4 and -s 1
+    out2 = tmp_path / "c"
+    out2.mkdir()
+    run(["-i", snap, "-o", str(out2), "-c", "4"])
+    a = _bigfile_field(box, d, pos[1], mas[1], table[1])
+    b = _bigfile_field(box, d, pos[4], mas[4], table[4])
+    compare(out2 / "PK-DMxst-PART_010", expected(box, d, [a, b], cross=True))
+    out3 = tmp_path / "s"
+    out3.mkdir()
+    run(["-i", snap, "-o", str(out3), "-s", "1"])
+    f0, M0 = _bigfile_field(box, d, pos[0], mas[0], table[0])
+    f4, M4 = _bigfile_field(box, d, pos[4], mas[4], table[4])
+    compare(out3 / "PK-by-PART_010", expected(box, d, [(f0 + f4, M0 + M4)]))
+    # a missing block ends the run with status 1
+    import shutil
+    shutil.rmtree(os.path.join(snap, "4", "Position"))
+    r = run(["-i", snap, "-o", str(out3)], ok=False)
+    assert r.returncode == 1 and "Failed to read from block" in r.stderr
