@@ -323,7 +323,7 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
         if (perm) perm[s0 + loc] = dbase[dg] + p;  // where this input record goes (coalesced)
       }
     }
-    __syncthreads();
+    // the next sub-chunk loads during the barrier and the write-out
     {
       const int64_t n0 = s0 + SUBN;
 #pragma unroll
@@ -332,6 +332,7 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
         if (i < ce) src.load(i, rec[it]);
       }
     }
+    __syncthreads();
     auto put = [&](int p) {
       const uint32_t dest = dbase[sdig[p]] + (uint32_t)p;
       float v[RW];
