@@ -151,6 +151,7 @@ struct Plan {
   // sort / deposit scratch
   DevBuf recA, recB, table1, table2, seginfo, tile_start, item_start, part_start, heavy, partials, item_tile;
   DevBuf permA, permB;      // sort permutations kept for the adjoint gather
+  DevBuf dig2;              // low tile-key digit of each pass-1 output record (one byte)
   DevBuf scan_tmp;
   RetainedOrder order;
 

@@ -168,6 +168,53 @@ __global__ void __launch_bounds__(SORT_THREADS) k_upsweep(Src<NM> src, ChunkMap 
     counts[tb + (int64_t)i * ts] = hist[i];
 }
 
+// k_upsweep's counts for a digit of at most 8 bits from the digit bytes the
+// previous downsweep wrote beside its output records (digs[i]: digit of
+// record i), so the records themselves are not read.  Reads the 4-byte words
+// covering [b, e): digs holds at least 4 bytes past the last record.
+__global__ void __launch_bounds__(SORT_THREADS) k_upsweep_bytes(const uint8_t *__restrict__ digs, ChunkMap cm,
+                                                                int nb, uint32_t *__restrict__ counts) {
+  constexpr int UNR = 4;
+  extern __shared__ uint32_t hist[];
+  int64_t b, e, tb, ts;
+  if (!cm.range(blockIdx.x, b, e, tb, ts)) return;
+  for (int i = threadIdx.x; i < nb; i += SORT_THREADS) hist[i] = 0;
+  __syncthreads();
+  const uint32_t *words = reinterpret_cast<const uint32_t *>(digs);
+  const int64_t w0 = b >> 2, w1 = (e + 3) >> 2;
+  for (int64_t base = w0; base < w1; base += SORT_THREADS * UNR) {
+    uint32_t v[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const int64_t w = base + u * SORT_THREADS + threadIdx.x;
+      v[u] = w < w1 ? words[w] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const int64_t w = base + u * SORT_THREADS + threadIdx.x;
+      // a word of four equal in-range digits adds them with one atomic, and
+      // a wavefront of such words of one digit with one atomic for all
+      const uint32_t d0 = v[u] & 0xffu;
+      const bool same4 = 4 * w >= b && 4 * w + 4 <= e && v[u] == d0 * 0x01010101u;
+      const uint32_t dw = __builtin_amdgcn_readfirstlane(d0);
+      if (__ballot(same4 && d0 == dw) == ~0ull) {
+        if ((threadIdx.x & 63) == 0) atomicAdd(&hist[dw], 256u);
+      } else if (same4) {
+        atomicAdd(&hist[d0], 4u);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int64_t i = 4 * w + j;
+          if (i >= b && i < e) atomicAdd(&hist[(v[u] >> (8 * j)) & 0xffu], 1u);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb; i += SORT_THREADS)
+    counts[tb + (int64_t)i * ts] = hist[i];
+}
+
 // Rank of each of a lane's PL records among the wave's records of its digit
 // in this sub-chunk, from LDS atomic adds that return the old count on the
 // wave's count row (a wave whose records share one digit adds them with one
@@ -260,7 +307,8 @@ template <int NM, int PL, bool DB>
 __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, float *__restrict__ dst, ChunkMap cm,
                                                                KeyGeom kg, DigitSel ds, int nb, int hw,
                                                                const uint32_t *__restrict__ offsets,
-                                                               uint32_t *__restrict__ perm) {
+                                                               uint32_t *__restrict__ perm,
+                                                               uint8_t *__restrict__ dig2, DigitSel ds2) {
   constexpr int RW = Src<NM>::RW;
   constexpr int SUBN = SORT_THREADS * PL;
   constexpr int NBUF = DB ? 2 : 1;
@@ -340,6 +388,7 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
       for (int k = 0; k < RW; k++) v[k] = stage[p * RW + k];
 #pragma unroll
       for (int k = 0; k < RW; k++) dst[(int64_t)dest * RW + k] = v[k];
+      if (dig2) dig2[dest] = (uint8_t)ds2.of(v, kg);
     };
     if (nsub == SUBN) {
 #pragma unroll
@@ -454,23 +503,24 @@ DsShape downsweep_shape(int nb, int rw) {
 
 template <int NM>
 void launch_downsweep(int64_t grid, hipStream_t s, Src<NM> src, float *dst, const ChunkMap &cm, const KeyGeom &kg,
-                      const DigitSel &ds, int nb, const uint32_t *offsets, uint32_t *perm) {
+                      const DigitSel &ds, int nb, const uint32_t *offsets, uint32_t *perm,
+                      uint8_t *dig2 = nullptr, const DigitSel &ds2 = DigitSel{0, 0, 0}) {
   GENPK_REQUIRE(nb <= 2 * SORT_THREADS, "sort digit too wide");
   const DsShape h = downsweep_shape(nb, Src<NM>::RW);
   const size_t lds = downsweep_lds(nb, Src<NM>::RW, h);
   const dim3 g((unsigned)grid), b(SORT_THREADS);
   if (h.pl == 8 && h.db)
-    hipLaunchKernelGGL((k_downsweep<NM, 8, true>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm);
+    hipLaunchKernelGGL((k_downsweep<NM, 8, true>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm, dig2, ds2);
   else if (h.pl == 8)
-    hipLaunchKernelGGL((k_downsweep<NM, 8, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm);
+    hipLaunchKernelGGL((k_downsweep<NM, 8, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm, dig2, ds2);
   else if (h.pl == 4 && h.db)
-    hipLaunchKernelGGL((k_downsweep<NM, 4, true>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm);
+    hipLaunchKernelGGL((k_downsweep<NM, 4, true>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm, dig2, ds2);
   else if (h.pl == 4)
-    hipLaunchKernelGGL((k_downsweep<NM, 4, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm);
+    hipLaunchKernelGGL((k_downsweep<NM, 4, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm, dig2, ds2);
   else if (h.pl == 2)
-    hipLaunchKernelGGL((k_downsweep<NM, 2, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm);
+    hipLaunchKernelGGL((k_downsweep<NM, 2, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm, dig2, ds2);
   else
-    hipLaunchKernelGGL((k_downsweep<NM, 1, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm);
+    hipLaunchKernelGGL((k_downsweep<NM, 1, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm, dig2, ds2);
   GENPK_LAUNCH_CHECK();
 }
 
@@ -517,6 +567,10 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     // pass 1: high digit over [0, n)
     uint32_t *t1 = static_cast<uint32_t *>(p.table1.ensure((size_t)nhi * g1 * 4));
     const DigitSel d1 = digit_sel(kg, tg.lo_bits, tg.hi_bits);
+    const DigitSel d2 = digit_sel(kg, 0, tg.lo_bits);
+    // a low digit of at most 8 bits is written as a byte beside each record
+    // of pass 1, and pass 2 counts those bytes instead of reading the records
+    uint8_t *dig2 = tg.lo_bits <= 8 ? static_cast<uint8_t *>(p.dig2.ensure((size_t)n + 64)) : nullptr;
     ChunkMap cm1{n, 0, nullptr, nullptr, g1, 0};
     {
       Stage st(p, "sort_upsweep1");
@@ -527,7 +581,7 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     exclusive_scan_u32(t1, t1, (int64_t)nhi * g1, p.scan_tmp, s);
     {
       Stage st(p, "sort_downsweep1");
-      launch_downsweep<NM>(g1, s, raw, A, cm1, kg, d1, nhi, t1, perm1);
+      launch_downsweep<NM>(g1, s, raw, A, cm1, kg, d1, nhi, t1, perm1, dig2, d2);
     }
     // segments of pass 2
     uint32_t *seg = static_cast<uint32_t *>(p.seginfo.ensure((size_t)(nhi + 1) * 3 * 4));
@@ -538,13 +592,15 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     exclusive_scan_u32(nchunks, chunk_start, nhi + 1, p.scan_tmp, s);
     const int64_t g2 = g1 + nhi;  // upper bound of pass-2 chunks
     uint32_t *t2 = static_cast<uint32_t *>(p.table2.ensure((size_t)g2 * nlo * 4));
-    const DigitSel d2 = digit_sel(kg, 0, tg.lo_bits);
     ChunkMap cm2{n, nhi, seg_start, chunk_start, 0, nlo};
     Src<NM> pa{A, nullptr, false};
     {
       Stage st(p, "sort_upsweep2");
-      hipLaunchKernelGGL((k_upsweep<NM>), dim3((unsigned)g2), dim3(SORT_THREADS), nlo * 4, s, pa, cm2, kg, d2, nlo,
-                         t2);
+      if (dig2)
+        hipLaunchKernelGGL(k_upsweep_bytes, dim3((unsigned)g2), dim3(SORT_THREADS), nlo * 4, s, dig2, cm2, nlo, t2);
+      else
+        hipLaunchKernelGGL((k_upsweep<NM>), dim3((unsigned)g2), dim3(SORT_THREADS), nlo * 4, s, pa, cm2, kg, d2, nlo,
+                           t2);
       GENPK_LAUNCH_CHECK();
     }
     // the written part of the table is its first (pass-2 chunks) * nlo entries;
