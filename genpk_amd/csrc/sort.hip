@@ -86,18 +86,24 @@ struct ChunkMap {
   }
 };
 
+// Tile key of a record with the cells along the axes not in `axes` (bit a:
+// axis a of x, y, z) taken as 0 and not computed.
+__device__ __forceinline__ uint32_t key_on_axes(const float *r, const KeyGeom &g, int axes) {
+  double fr;
+  const int cx = (axes & 1) ? cell_of(r[0], g.units, g.d, fr) : 0;
+  const int cy = (axes & 2) ? cell_of(r[1], g.units, g.d, fr) : 0;
+  const int cz = (axes & 4) ? cell_of(r[2], g.units, g.d, fr) : 0;
+  return (uint32_t)((((cx >> g.sx) * g.nty + (cy >> g.sy)) * g.ntz) + (cz >> g.sz));
+}
+
 struct DigitSel {
   int shift;
   uint32_t mask;
   int axes;  // bit a set: the digit depends on the tile index along axis a (x, y, z)
+  __device__ __forceinline__ uint32_t of_key(uint32_t key) const { return (key >> shift) & mask; }
   // digit of a record; cells along axes the digit does not depend on are not computed
   __device__ __forceinline__ uint32_t of(const float *r, const KeyGeom &g) const {
-    double fr;
-    const int cx = (axes & 1) ? cell_of(r[0], g.units, g.d, fr) : 0;
-    const int cy = (axes & 2) ? cell_of(r[1], g.units, g.d, fr) : 0;
-    const int cz = (axes & 4) ? cell_of(r[2], g.units, g.d, fr) : 0;
-    const uint32_t key = (uint32_t)((((cx >> g.sx) * g.nty + (cy >> g.sy)) * g.ntz) + (cz >> g.sz));
-    return (key >> shift) & mask;
+    return of_key(key_on_axes(r, g, axes));
   }
 };
 
@@ -222,17 +228,22 @@ __global__ void __launch_bounds__(SORT_THREADS) k_upsweep_bytes(const uint8_t *_
 // one instruction's lanes and, where waves share a count row, by the order
 // in which they run: the order of the records within a tile is not
 // specified (the tile kernels do not depend on it).
+// With `two`, lo also receives each record's digit of ds2 (computed from the
+// same cells).
 template <int NM, int PL>
 __device__ __forceinline__ void rank_records(const float (&rec)[PL][Src<NM>::RW], int wbase, int nsub,
-                                             const KeyGeom &kg, const DigitSel &ds, uint32_t *wrow,
-                                             uint32_t (&dig)[PL], uint32_t (&rnk)[PL]) {
+                                             const KeyGeom &kg, const DigitSel &ds, const DigitSel &ds2, bool two,
+                                             uint32_t *wrow, uint32_t (&dig)[PL], uint32_t (&lo)[PL],
+                                             uint32_t (&rnk)[PL]) {
   const int lane = threadIdx.x & 63;
   uint64_t gpeers[PL];
   int glead[PL];
 #pragma unroll
   for (int it = 0; it < PL; it++) {
     const bool valid = wbase + it * 64 < nsub;
-    const uint32_t dg = valid ? ds.of(rec[it], kg) : 0u;
+    const uint32_t key = valid ? key_on_axes(rec[it], kg, two ? ds.axes | ds2.axes : ds.axes) : 0u;
+    const uint32_t dg = ds.of_key(key);
+    lo[it] = ds2.of_key(key);
     const uint64_t vmask = __ballot(valid);
     const int lead = (int)__builtin_ctzll(vmask | (1ull << 63));
     const uint32_t d0 = __builtin_amdgcn_readlane(dg, lead);
@@ -324,6 +335,8 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
   int64_t cb, ce, tb, ts;
   if (!cm.range(blockIdx.x, cb, ce, tb, ts)) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // with dig2 (digits of at most 8 bits), sdig holds the digit and the ds2 digit
+  const bool two = dig2 != nullptr;
   for (int i = threadIdx.x; i < nb; i += SORT_THREADS) gbase[i] = offsets[tb + (int64_t)i * ts];
   for (int i = threadIdx.x; i < NBUF * hw * nb; i += SORT_THREADS) whist_all[i] = 0;
   const int row = (wave % hw) * nb;
@@ -341,8 +354,8 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
   for (int64_t s0 = cb; s0 < ce; s0 += SUBN, sub++) {
     const int nsub = (int)min((int64_t)SUBN, ce - s0);
     uint32_t *whist = whist_all + (DB ? (sub & 1) * hw * nb : 0);
-    uint32_t dig[PL], rnk[PL];
-    rank_records<NM, PL>(rec, wbase, nsub, kg, ds, whist + row, dig, rnk);
+    uint32_t dig[PL], lo[PL], rnk[PL];
+    rank_records<NM, PL>(rec, wbase, nsub, kg, ds, ds2, two, whist + row, dig, lo, rnk);
     __syncthreads();
     {
       uint32_t total = 0;
@@ -367,7 +380,7 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
         const uint32_t p = lstart[dg] + whist[row + dg] + rnk[it];
 #pragma unroll
         for (int k = 0; k < RW; k++) stage[p * RW + k] = rec[it][k];
-        sdig[p] = (uint16_t)dg;
+        sdig[p] = (uint16_t)(two ? dg | lo[it] << 8 : dg);
         if (perm) perm[s0 + loc] = dbase[dg] + p;  // where this input record goes (coalesced)
       }
     }
@@ -382,13 +395,14 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
     }
     __syncthreads();
     auto put = [&](int p) {
-      const uint32_t dest = dbase[sdig[p]] + (uint32_t)p;
+      const uint32_t sd = sdig[p];
+      const uint32_t dest = dbase[two ? sd & 0xffu : sd] + (uint32_t)p;
       float v[RW];
 #pragma unroll
       for (int k = 0; k < RW; k++) v[k] = stage[p * RW + k];
 #pragma unroll
       for (int k = 0; k < RW; k++) dst[(int64_t)dest * RW + k] = v[k];
-      if (dig2) dig2[dest] = (uint8_t)ds2.of(v, kg);
+      if (two) dig2[dest] = (uint8_t)(sd >> 8);
     };
     if (nsub == SUBN) {
 #pragma unroll
@@ -506,21 +520,28 @@ void launch_downsweep(int64_t grid, hipStream_t s, Src<NM> src, float *dst, cons
                       const DigitSel &ds, int nb, const uint32_t *offsets, uint32_t *perm,
                       uint8_t *dig2 = nullptr, const DigitSel &ds2 = DigitSel{0, 0, 0}) {
   GENPK_REQUIRE(nb <= 2 * SORT_THREADS, "sort digit too wide");
+  GENPK_REQUIRE(!dig2 || (nb <= 256 && ds2.mask <= 255u), "sort digit bytes need digits of at most 8 bits");
   const DsShape h = downsweep_shape(nb, Src<NM>::RW);
   const size_t lds = downsweep_lds(nb, Src<NM>::RW, h);
   const dim3 g((unsigned)grid), b(SORT_THREADS);
   if (h.pl == 8 && h.db)
-    hipLaunchKernelGGL((k_downsweep<NM, 8, true>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm, dig2, ds2);
+    hipLaunchKernelGGL((k_downsweep<NM, 8, true>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
+                       dig2, ds2);
   else if (h.pl == 8)
-    hipLaunchKernelGGL((k_downsweep<NM, 8, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm, dig2, ds2);
+    hipLaunchKernelGGL((k_downsweep<NM, 8, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
+                       dig2, ds2);
   else if (h.pl == 4 && h.db)
-    hipLaunchKernelGGL((k_downsweep<NM, 4, true>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm, dig2, ds2);
+    hipLaunchKernelGGL((k_downsweep<NM, 4, true>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
+                       dig2, ds2);
   else if (h.pl == 4)
-    hipLaunchKernelGGL((k_downsweep<NM, 4, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm, dig2, ds2);
+    hipLaunchKernelGGL((k_downsweep<NM, 4, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
+                       dig2, ds2);
   else if (h.pl == 2)
-    hipLaunchKernelGGL((k_downsweep<NM, 2, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm, dig2, ds2);
+    hipLaunchKernelGGL((k_downsweep<NM, 2, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
+                       dig2, ds2);
   else
-    hipLaunchKernelGGL((k_downsweep<NM, 1, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm, dig2, ds2);
+    hipLaunchKernelGGL((k_downsweep<NM, 1, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
+                       dig2, ds2);
   GENPK_LAUNCH_CHECK();
 }
 
@@ -568,9 +589,11 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     uint32_t *t1 = static_cast<uint32_t *>(p.table1.ensure((size_t)nhi * g1 * 4));
     const DigitSel d1 = digit_sel(kg, tg.lo_bits, tg.hi_bits);
     const DigitSel d2 = digit_sel(kg, 0, tg.lo_bits);
-    // a low digit of at most 8 bits is written as a byte beside each record
-    // of pass 1, and pass 2 counts those bytes instead of reading the records
-    uint8_t *dig2 = tg.lo_bits <= 8 ? static_cast<uint8_t *>(p.dig2.ensure((size_t)n + 64)) : nullptr;
+    // with digits of at most 8 bits, pass 1 writes each record's low digit as
+    // a byte beside it, and pass 2 counts those bytes instead of reading the
+    // records
+    uint8_t *dig2 = tg.lo_bits <= 8 && tg.hi_bits <= 8 ? static_cast<uint8_t *>(p.dig2.ensure((size_t)n + 64))
+                                                        : nullptr;
     ChunkMap cm1{n, 0, nullptr, nullptr, g1, 0};
     {
       Stage st(p, "sort_upsweep1");
