@@ -150,7 +150,14 @@ struct BinTrack {
 
 // Fused final FFT pass along x + deconvolved binning.  NF fields (1: auto,
 // 2: auto a, auto b and cross a.b*).  Persistent over column blocks.
-template <typename T, int NF>
+//
+// REG1 (one field, ping-pong buffers, n * B = BIN_THREADS * MAXE, a first
+// stage of radix MAXE and an even number of stages): each thread's MAXE
+// loaded elements are the inputs of its butterfly in the first stage, which
+// then runs from registers into buf2.  The previous block no longer reads
+// buf2 (an even number of stages leaves its result in buf), so no barrier
+// comes before that stage.
+template <typename T, int NF, bool REG1>
 __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__restrict__ fa,
                                                             typename Cx<T>::C *__restrict__ fb, BinGeom g, int B,
                                                             int nkb, RadixPlan rp,
@@ -208,22 +215,28 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
     const int nb = (int)min((int64_t)B, hz - kz0);
     const int64_t off = y * hz + kz0;
     load(v);
-    __syncthreads();  // the previous block's binning is done with buf
-#pragma unroll
-    for (int q = 0; q < MAXE; q++) {
-      const int i = threadIdx.x + q * BIN_THREADS;
-      if (i < tot) {
-        buf[i] = va[q];
-        if (NF == 2) buf[tot + i] = vb[NF == 2 ? q : 0];
-      }
-    }
-    __syncthreads();
     C *res = buf;
-    if (NF == 1 && pp) {
-      res = lds_fft_pp<-1, true, BinCfg<T>::MAXE>(buf, buf2, rp, B, lgb, B, 1, stw);
+    if constexpr (REG1) {
+      fft_first_regs<MAXE, -1>(va, buf2, B, lgb, B, 1);
+      __syncthreads();
+      res = lds_fft_pp<-1, true, MAXE>(buf2, buf, rp, B, lgb, B, 1, stw, 1);
     } else {
-      lds_fft<-1, true, BinCfg<T>::MAXE>(buf, rp, B, lgb, B, 1, stw);
-      if (NF == 2) lds_fft<-1, true, BinCfg<T>::MAXE>(buf + n * B, rp, B, lgb, B, 1, stw);
+      __syncthreads();  // the previous block's binning is done with buf
+#pragma unroll
+      for (int q = 0; q < MAXE; q++) {
+        const int i = threadIdx.x + q * BIN_THREADS;
+        if (i < tot) {
+          buf[i] = va[q];
+          if (NF == 2) buf[tot + i] = vb[NF == 2 ? q : 0];
+        }
+      }
+      __syncthreads();
+      if (NF == 1 && pp) {
+        res = lds_fft_pp<-1, true, MAXE>(buf, buf2, rp, B, lgb, B, 1, stw);
+      } else {
+        lds_fft<-1, true, MAXE>(buf, rp, B, lgb, B, 1, stw);
+        if (NF == 2) lds_fft<-1, true, MAXE>(buf + n * B, rp, B, lgb, B, 1, stw);
+      }
     }
     if (keep) {
       for (int i = threadIdx.x; i < n * B; i += BIN_THREADS) {
@@ -480,18 +493,26 @@ static size_t fused_lds(const Plan &a, int B, int nrbins, bool pp) {
          (size_t)a.d * 4 + (size_t)(nrbins + 1) * 4;
 }
 
+// Whether the REG1 instance applies (one field with ping-pong buffers).
+template <typename T>
+static bool fused_reg1(const Plan &a, int B, bool pp) {
+  const RadixPlan rp = make_radix_plan((int)a.d);
+  return pp && rp.st[0].R == BinCfg<T>::MAXE && a.d * B == (int64_t)BIN_THREADS * BinCfg<T>::MAXE && rp.nst % 2 == 0;
+}
+
 // Persistent grid of the fused kernel: as many workgroups as stay resident
 // (occupancy of this instance x CUs), at most one per column block.
-template <typename T, int NF>
+template <typename T, int NF, bool REG1>
 static int fused_grid(const Plan &a, int B, int nrbins, bool pp) {
   int per_cu = 0;
-  GENPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_xpass_bin<T, NF>),
+  GENPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
+                                                          reinterpret_cast<const void *>(&k_xpass_bin<T, NF, REG1>),
                                                           BIN_THREADS, fused_lds<T, NF>(a, B, nrbins, pp)));
   const int64_t nblk = a.d * ceil_div(a.hz, B);
   return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)std::max(per_cu, 1) * num_cus(), nblk));
 }
 
-template <typename T, int NF>
+template <typename T, int NF, bool REG1>
 static void launch_fused(Plan &a, Plan *b, const BinTables &bt, int B, bool keep, bool pp, double *part, int G) {
   using C = typename Cx<T>::C;
   constexpr int NS = NF == 1 ? 1 : 3;
@@ -501,7 +522,7 @@ static void launch_fused(Plan &a, Plan *b, const BinTables &bt, int B, bool keep
   BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), bt.thr.as<int64_t>(), a.wtab.as<float>(), bt.lq};
   const size_t lds = fused_lds<T, NF>(a, B, bt.nrbins, pp);
   Stage st(a, "xpass_bin");
-  hipLaunchKernelGGL((k_xpass_bin<T, NF>), dim3(G), dim3(BIN_THREADS), lds, a.stream, a.field.as<C>(),
+  hipLaunchKernelGGL((k_xpass_bin<T, NF, REG1>), dim3(G), dim3(BIN_THREADS), lds, a.stream, a.field.as<C>(),
                      b ? b->field.as<C>() : (C *)nullptr, g, B, nkb, rp, tw, keep ? 1 : 0, pp ? 1 : 0, part);
   GENPK_LAUNCH_CHECK();
 }
@@ -535,17 +556,22 @@ void bin_spectra(Plan &a, Plan *b, int nrbins, bool fused_xpass, double *pow_aa,
     // one field: ping-pong FFT buffers where the LDS has room for them
     const bool pp = !b && (a.prec == 4 ? fused_lds<float, 1>(a, B, nrbins, true)
                                        : fused_lds<double, 1>(a, B, nrbins, true)) <= (size_t)LDS_LIMIT;
+    const bool reg1 = !b && (a.prec == 4 ? fused_reg1<float>(a, B, pp) : fused_reg1<double>(a, B, pp));
     if (a.prec == 4)
-      G = b ? fused_grid<float, 2>(a, B, nrbins, false) : fused_grid<float, 1>(a, B, nrbins, pp);
+      G = b ? fused_grid<float, 2, false>(a, B, nrbins, false)
+            : reg1 ? fused_grid<float, 1, true>(a, B, nrbins, pp) : fused_grid<float, 1, false>(a, B, nrbins, pp);
     else
-      G = b ? fused_grid<double, 2>(a, B, nrbins, false) : fused_grid<double, 1>(a, B, nrbins, pp);
+      G = b ? fused_grid<double, 2, false>(a, B, nrbins, false)
+            : reg1 ? fused_grid<double, 1, true>(a, B, nrbins, pp) : fused_grid<double, 1, false>(a, B, nrbins, pp);
     part = static_cast<double *>(a.bin_part.ensure((size_t)G * ns * nrbins * 8));
     if (a.prec == 4) {
-      if (b) launch_fused<float, 2>(a, b, bt, B, false, false, part, G);
-      else launch_fused<float, 1>(a, b, bt, B, false, pp, part, G);
+      if (b) launch_fused<float, 2, false>(a, b, bt, B, false, false, part, G);
+      else if (reg1) launch_fused<float, 1, true>(a, b, bt, B, false, pp, part, G);
+      else launch_fused<float, 1, false>(a, b, bt, B, false, pp, part, G);
     } else {
-      if (b) launch_fused<double, 2>(a, b, bt, B, false, false, part, G);
-      else launch_fused<double, 1>(a, b, bt, B, false, pp, part, G);
+      if (b) launch_fused<double, 2, false>(a, b, bt, B, false, false, part, G);
+      else if (reg1) launch_fused<double, 1, true>(a, b, bt, B, false, pp, part, G);
+      else launch_fused<double, 1, false>(a, b, bt, B, false, pp, part, G);
     }
     // the field keeps the forward state after the z and y passes (an adjoint can start from it)
     a.fstate = FieldState::YDone;

@@ -260,19 +260,33 @@ __device__ __forceinline__ void lds_fft(C *buf, const RadixPlan &rp, int nbatch,
     fft_stage_r<SIGN, BFAST, MAXE, false>(rp.st[s].R, buf, buf, nbatch, lgb, se, sb, rp.st[s], tw);
 }
 
-// Full transform alternating between buffers a and b (one barrier per
-// stage); returns the buffer that holds the result.  The caller has filled a
-// and synchronised.
+// Stages s0.. of a transform alternating between buffers a and b (one
+// barrier per stage); returns the buffer that holds the result.  The caller
+// has filled a and synchronised.
 template <int SIGN, bool BFAST, int MAXE, typename C>
 __device__ __forceinline__ C *lds_fft_pp(C *a, C *b, const RadixPlan &rp, int nbatch, int lgb, int se, int sb,
-                                         const C *__restrict__ tw) {
-  for (int s = 0; s < rp.nst; s++) {
+                                         const C *__restrict__ tw, int s0 = 0) {
+  for (int s = s0; s < rp.nst; s++) {
     fft_stage_r<SIGN, BFAST, MAXE, true>(rp.st[s].R, a, b, nbatch, lgb, se, sb, rp.st[s], tw);
     C *t = a;
     a = b;
     b = t;
   }
   return a;
+}
+
+// First stage (every twiddle 1) of a radix-R transform whose R inputs this
+// thread holds in v: elements j + r * nq (nq = n / R) of transform b, with
+// b = threadIdx.x & (nbatch - 1) and j = threadIdx.x >> lgb (one butterfly
+// per thread, batch index fastest, as fft_stage<R, SIGN, true>).  Writes the
+// outputs to out, with no barrier.
+template <int R, int SIGN, typename C>
+__device__ __forceinline__ void fft_first_regs(C (&v)[R], C *out, int nbatch, int lgb, int se, int sb) {
+  Dft<R, SIGN, C>::run(v);
+  const int b = threadIdx.x & (nbatch - 1), j = threadIdx.x >> lgb;
+  C *dst = out + b * sb + j * R * se;
+#pragma unroll
+  for (int r = 0; r < R; r++) dst[r * se] = v[r];
 }
 
 __host__ __device__ inline int ilog2(int64_t v) {
