@@ -33,7 +33,9 @@ template <> struct FftCfg<double> { static constexpr int ELEMS = 4096; static co
 // loading, halo cells folded in (the fold), so the real-space grid is never
 // written.
 // The LDS footprint (up to 64 KB) allows two workgroups per CU, so the
-// register budget of two waves per SIMD is available.
+// register budget of two waves per SIMD is available.  The rows sit in LDS
+// with one element of padding per 32 (lds_ix), which spreads the strided
+// writes of the FFT stages over the banks.
 template <typename T, bool FROM_TILES>
 __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(1, 2))) k_r2c_z(T *__restrict__ field, const T *__restrict__ tiles,
                                                       FoldGeom fg, FastDiv fdd, int64_t rowlen, int64_t nrows, int L,
@@ -109,15 +111,15 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
     const int i = threadIdx.x + q * FFT_THREADS;
     // z-halo added after all loads are in flight (a + b rounds once, as fold_value)
     if (FROM_TILES) v[q].x = v[q].x + zh[FROM_TILES ? q : 0];
-    if (i < tot) buf[i] = v[q];
+    if (i < tot) buf[lds_ix<true>(i)] = v[q];
   }
   // twiddles of the M-point FFT and of the even/odd split, staged in LDS
-  C *stw = buf + (size_t)L * M;   // [M]
+  C *stw = buf + lds_len<true>((int64_t)L * M);  // [M]
   C *sxw = stw + M;               // [M+1]
   for (int i = threadIdx.x; i < M; i += FFT_THREADS) stw[i] = twM[i];
   for (int i = threadIdx.x; i <= M; i += FFT_THREADS) sxw[i] = twd[i];
   __syncthreads();
-  lds_fft<-1, false, MAXE>(buf, rp, L, 0, 1, M, stw);
+  lds_fft<-1, false, MAXE, C, true>(buf, rp, L, 0, 1, M, stw);
   // X_k = (Z_k + conj Z_{M-k})/2 - i/2 * w^k (Z_k - conj Z_{M-k}),  w = exp(-2 pi i / d)
   auto split = [&](C zk, C zc, C w) {
     const C e = C{(zk.x + zc.x) * (T)0.5, (zk.y + zc.y) * (T)0.5};
@@ -131,8 +133,8 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
 #pragma unroll
     for (int l = 0; l < MAXE; l++) {
       if (l < nl) {
-        const C zk = buf[l * M + k];
-        const C zc = cconj(buf[l * M + (k == 0 ? 0 : M - k)]);
+        const C zk = buf[lds_ix<true>(l * M + k)];
+        const C zc = cconj(buf[lds_ix<true>(l * M + (k == 0 ? 0 : M - k))]);
         C *row = reinterpret_cast<C *>(field + (row0 + l) * rowlen);
         row[k] = split(zk, zc, w);
         if (k == 0) row[M] = split(zk, zc, wm);
@@ -143,8 +145,8 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
   const int nout = M + 1;
   for (int i = threadIdx.x; i < nl * nout; i += FFT_THREADS) {
     const int l = (int)fdM1.div((uint32_t)i), k = i - l * nout;
-    const C zk = buf[l * M + (k == M ? 0 : k)];
-    const C zc = cconj(buf[l * M + (k == 0 ? 0 : M - k)]);
+    const C zk = buf[lds_ix<true>(l * M + (k == M ? 0 : k))];
+    const C zc = cconj(buf[lds_ix<true>(l * M + (k == 0 ? 0 : M - k))]);
     const C x = split(zk, zc, sxw[k]);
     T *row = field + (row0 + l) * rowlen;
     row[2 * k] = x.x;
@@ -318,7 +320,7 @@ static void r2c_z_impl(Plan &p, bool from_tiles) {
   const C *twd = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
   const FoldGeom fg = fold_geom(p);
   const dim3 grid((unsigned)ceil_div(nrows, L));
-  const size_t lds = ((size_t)L * M + 2 * M + 1) * sizeof(C);
+  const size_t lds = ((size_t)lds_len<true>((int64_t)L * M) + 2 * M + 1) * sizeof(C);
   if (from_tiles) {
     Stage st(p, "fold_fft_z");
     hipLaunchKernelGGL((k_r2c_z<T, true>), grid, dim3(FFT_THREADS), lds, p.stream, p.field.as<T>(),

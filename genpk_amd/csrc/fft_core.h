@@ -180,13 +180,20 @@ inline bool fft_length_ok(int64_t n) {
   return make_radix_plan((int)n).nst >= 0;
 }
 
+// LDS position of element index i: PAD inserts one element after every 32,
+// so that accesses at a stride of R elements (the writes of a Stockham
+// stage) spread over the LDS banks.
+template <bool PAD> __device__ __forceinline__ int lds_ix(int i) { return PAD ? i + (i >> 5) : i; }
+template <bool PAD> __host__ __device__ inline int64_t lds_len(int64_t n) { return PAD ? n + (n >> 5) + 1 : n; }
+
 // One Stockham stage of radix R on nbatch transforms held in LDS, element
-// e of transform b at in[e*se + b*sb], results to out (same layout).  MAXE
-// bounds the elements a thread owns per stage.  BFAST: the batch index
-// varies fastest across threads (interleaved layout, nbatch = 1 << lgb);
-// otherwise the element index does.  In place (in == out) all reads pass a
-// barrier before the writes; out of place (PP) one barrier ends the stage.
-template <int R, int SIGN, bool BFAST, int MAXE, bool PP, typename C>
+// e of transform b at in[e*se + b*sb], results to out (same layout; with
+// PAD, index i at lds_ix(i)).  MAXE bounds the elements a thread owns per
+// stage.  BFAST: the batch index varies fastest across threads
+// (interleaved layout, nbatch = 1 << lgb); otherwise the element index
+// does.  In place (in == out) all reads pass a barrier before the writes;
+// out of place (PP) one barrier ends the stage.
+template <int R, int SIGN, bool BFAST, int MAXE, bool PP, typename C, bool PAD = false>
 __device__ __forceinline__ void fft_stage(const C *in, C *out, int nbatch, int lgb, int se, int sb,
                                           const StagePlan &sp, const C *__restrict__ tw) {
   constexpr int MAXB = (MAXE + R - 1) / R;
@@ -208,18 +215,22 @@ __device__ __forceinline__ void fft_stage(const C *in, C *out, int nbatch, int l
       }
       const int jq = (int)sp.fns.div((uint32_t)j);
       const int k = j - jq * ns;
-      const C *src = in + b * sb + j * se;
-      v[i][0] = src[0];
+      const int i0 = b * sb + j * se;
+      auto src = [&](int r) -> C {
+        if constexpr (PAD) return in[lds_ix<true>(i0 + r * nq * se)];
+        else return in[i0 + r * nq * se];
+      };
+      v[i][0] = src(0);
       if (ns == 1) {
         // first stage: every twiddle is 1
 #pragma unroll
-        for (int r = 1; r < R; r++) v[i][r] = src[r * nq * se];
+        for (int r = 1; r < R; r++) v[i][r] = src(r);
       } else {
 #pragma unroll
         for (int r = 1; r < R; r++) {
           C t = tw[r * k * sp.twstep];
           if (SIGN > 0) t.y = -t.y;
-          v[i][r] = cmul(src[r * nq * se], t);
+          v[i][r] = cmul(src(r), t);
         }
       }
       dsto[i] = b * sb + (jq * ns * R + k) * se;
@@ -231,33 +242,35 @@ __device__ __forceinline__ void fft_stage(const C *in, C *out, int nbatch, int l
     const int u = threadIdx.x + i * blockDim.x;
     if (u < nbf) {
       Dft<R, SIGN, C>::run(v[i]);
-      C *dst = out + dsto[i];
 #pragma unroll
-      for (int r = 0; r < R; r++) dst[r * ns * se] = v[i][r];
+      for (int r = 0; r < R; r++) {
+        if constexpr (PAD) out[lds_ix<true>(dsto[i] + r * ns * se)] = v[i][r];
+        else out[dsto[i] + r * ns * se] = v[i][r];
+      }
     }
   }
   __syncthreads();
 }
 
-template <int SIGN, bool BFAST, int MAXE, bool PP, typename C>
+template <int SIGN, bool BFAST, int MAXE, bool PP, typename C, bool PAD = false>
 __device__ __forceinline__ void fft_stage_r(int R, const C *in, C *out, int nbatch, int lgb, int se, int sb,
                                             const StagePlan &sp, const C *__restrict__ tw) {
   switch (R) {
-    case 8: fft_stage<8, SIGN, BFAST, MAXE, PP>(in, out, nbatch, lgb, se, sb, sp, tw); break;
-    case 4: fft_stage<4, SIGN, BFAST, MAXE, PP>(in, out, nbatch, lgb, se, sb, sp, tw); break;
-    case 2: fft_stage<2, SIGN, BFAST, MAXE, PP>(in, out, nbatch, lgb, se, sb, sp, tw); break;
-    case 3: fft_stage<3, SIGN, BFAST, MAXE, PP>(in, out, nbatch, lgb, se, sb, sp, tw); break;
-    case 5: fft_stage<5, SIGN, BFAST, MAXE, PP>(in, out, nbatch, lgb, se, sb, sp, tw); break;
+    case 8: fft_stage<8, SIGN, BFAST, MAXE, PP, C, PAD>(in, out, nbatch, lgb, se, sb, sp, tw); break;
+    case 4: fft_stage<4, SIGN, BFAST, MAXE, PP, C, PAD>(in, out, nbatch, lgb, se, sb, sp, tw); break;
+    case 2: fft_stage<2, SIGN, BFAST, MAXE, PP, C, PAD>(in, out, nbatch, lgb, se, sb, sp, tw); break;
+    case 3: fft_stage<3, SIGN, BFAST, MAXE, PP, C, PAD>(in, out, nbatch, lgb, se, sb, sp, tw); break;
+    case 5: fft_stage<5, SIGN, BFAST, MAXE, PP, C, PAD>(in, out, nbatch, lgb, se, sb, sp, tw); break;
     default: break;
   }
 }
 
 // Full transform in place; the caller has filled buf and synchronised.
-template <int SIGN, bool BFAST, int MAXE, typename C>
+template <int SIGN, bool BFAST, int MAXE, typename C, bool PAD = false>
 __device__ __forceinline__ void lds_fft(C *buf, const RadixPlan &rp, int nbatch, int lgb, int se, int sb,
                                         const C *__restrict__ tw) {
   for (int s = 0; s < rp.nst; s++)
-    fft_stage_r<SIGN, BFAST, MAXE, false>(rp.st[s].R, buf, buf, nbatch, lgb, se, sb, rp.st[s], tw);
+    fft_stage_r<SIGN, BFAST, MAXE, false, C, PAD>(rp.st[s].R, buf, buf, nbatch, lgb, se, sb, rp.st[s], tw);
 }
 
 // Stages s0.. of a transform alternating between buffers a and b (one
