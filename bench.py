@@ -52,8 +52,13 @@ def stage_bytes(name, n, c, p_pad, geo, rec):
     read and write once for its function.  geo: deposit tiling of the plan
     (tiles K, halo'd cells per tile, primary cells per tile)."""
     K, cells, prim = geo
+    digit_bytes = sort_digit_bytes(K)
+    if name == "sort_upsweep2" and digit_bytes:
+        return n                                         # read the low digits (one byte each)
     if name.startswith("sort_upsweep"):
         return rec * n                                   # read the records
+    if name == "sort_downsweep1" and digit_bytes:
+        return 2 * rec * n + n                           # read + write the records, write the low digits
     if name.startswith("sort_downsweep"):
         return 2 * rec * n                               # read + write the records
     if name == "deposit_tiles":
@@ -70,6 +75,18 @@ def stage_bytes(name, n, c, p_pad, geo, rec):
     if name == "adjoint_gather":
         return (rec + 4) * n + c * K * cells + 12 * n    # records with index, G tiles, gradient write
     return 0
+
+
+def sort_digit_bytes(K):
+    """Whether the tile sort of K tiles runs two passes with digits of at
+    most 8 bits, its first pass then writing each record's low digit as a
+    byte (the key of ceil(log2 K) bits splits into a high digit and a low
+    digit of half the bits, rounded up)."""
+    kbits = max(int(K - 1).bit_length(), 0)
+    if kbits <= 9:
+        return False
+    lo = (kbits + 1) // 2
+    return lo <= 8 and kbits - lo <= 8
 
 
 def tile_geometry(d):

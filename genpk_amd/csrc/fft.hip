@@ -154,6 +154,12 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
   }
 }
 
+// Elements of the inverse z pass's row buffer: L rows of M + 1 while the
+// half spectrum is unfolded, then L padded rows of M for the transform.
+__host__ __device__ inline int64_t c2r_span(int64_t L, int64_t M) {
+  return L * (M + 1) > lds_len<true>(L * M) ? L * (M + 1) : lds_len<true>(L * M);
+}
+
 // z pass, inverse: d/2+1 complex -> d reals (unnormalised c2r); the
 // imaginary parts of kz = 0 and kz = d/2 do not contribute.
 template <typename T>
@@ -196,18 +202,19 @@ __global__ void __launch_bounds__(FFT_THREADS) k_c2r_z(T *__restrict__ field, in
     }
   }
   __syncthreads();
+  // the transform runs on padded rows (lds_ix), as the forward z pass
 #pragma unroll
   for (int q = 0; q < MAXE; q++) {
     const int i = threadIdx.x + q * FFT_THREADS;
-    if (i < L * M) buf[i] = z[q];
+    if (i < L * M) buf[lds_ix<true>(i)] = z[q];
   }
-  C *stw = buf + (size_t)L * (M + 1);
+  C *stw = buf + c2r_span(L, M);
   for (int i = threadIdx.x; i < M; i += FFT_THREADS) stw[i] = twM[i];
   __syncthreads();
-  lds_fft<+1, false, MAXE>(buf, rp, L, 0, 1, M, stw);
+  lds_fft<+1, false, MAXE, C, true>(buf, rp, L, 0, 1, M, stw);
   for (int i = threadIdx.x; i < nl * M; i += FFT_THREADS) {
     const int l = (int)fdM.div((uint32_t)i), m = i - l * M;
-    const C v = buf[i];
+    const C v = buf[lds_ix<true>(i)];
     T *row = field + (row0 + l) * rowlen;
     row[2 * m] = v.x;
     row[2 * m + 1] = v.y;
@@ -346,7 +353,7 @@ static void c2r_z_impl(Plan &p) {
   const C *twd = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
   Stage st(p, "ifft_z");
   hipLaunchKernelGGL(k_c2r_z<T>, dim3((unsigned)ceil_div(nrows, L)), dim3(FFT_THREADS),
-                     ((size_t)L * (M + 1) + M) * sizeof(C), p.stream, p.field.as<T>(), p.rowlen, nrows, L,
+                     ((size_t)c2r_span(L, M) + M) * sizeof(C), p.stream, p.field.as<T>(), p.rowlen, nrows, L,
                      FastDiv((uint32_t)M), FastDiv((uint32_t)(M + 1)), rp, twM, twd);
   GENPK_LAUNCH_CHECK();
 }
