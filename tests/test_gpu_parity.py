@@ -123,6 +123,40 @@ def test_deposit_vs_oracle(gpu, d, n, with_masses, prec):
     assert abs(got.sum() - total) <= 1e-6 * total
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("with_masses", [False, True])
+def test_deposit_wide_key_vs_oracle(gpu, with_masses, prec):
+    """d = 1280: 80 x 40 x 40 = 128000 tiles, a 17-bit key whose 9-bit low
+    digit takes the second sort pass's histogram from the records (not from
+    one-byte digits).  The particles lie in cells [0, 100) of each axis, so
+    no weight wraps and the oracle deposits them on a 128^3 grid of the same
+    cell size (box scaled by 128 / 1280: the same units d / box)."""
+    import torch
+    announce("deposit wide key d=1280", with_masses, prec)
+    d, box, dsmall, n = 1280, 50.0, 128, 300000
+    rng = np.random.default_rng(1280 + with_masses)
+    cells = rng.random((n, 3)) * 100.0
+    pos = (cells * (box / d)).astype(np.float32)
+    masses = (rng.random(n) + 0.5).astype(np.float32) if with_masses else None
+    box_small = box * dsmall / d
+    assert (np.float64(d) / box) == (np.float64(dsmall) / box_small)
+    p = genpk_amd.Plan(d, prec)
+    p.deposit(to_dev(torch, pos), box, None if masses is None else to_dev(torch, masses), 1.3)
+    dens = p.export_density()
+    got = dens[:dsmall, :dsmall, :dsmall].cpu().numpy().astype(np.float64)
+    full_sum = float(dens[:, :, :d].sum(dtype=torch.float64).cpu())
+    del dens, p
+    torch.cuda.empty_cache()
+    ref = oracle.new_field(dsmall)
+    oracle.fieldize(box_small, dsmall, ref, pos, masses, 1.3, 1)
+    ref = ref[:, :, :dsmall]
+    tol = 1e-9 if prec == "f64" else 2e-7
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
+    total = masses.astype(np.float64).sum() if with_masses else 1.3 * n
+    assert abs(full_sum - total) <= 1e-6 * total
+    assert abs(got.sum() - total) <= 1e-6 * total  # nothing outside the compared corner
+
+
 def test_deposit_heavy_tile_split(gpu):
     import torch
     announce("deposit heavy tile d=64 n=400000 clumped")
