@@ -235,8 +235,8 @@ def cpu_baseline(pos_dev, box, dims, nbins, n_total, sample_log2):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU decomposition is "
-              "described in DESIGN.md and not built.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU slab decomposition "
+              "is designed but not built.", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

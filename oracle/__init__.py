@@ -14,9 +14,10 @@ restates:
 * ``fft_r2c_3d``      -- FP64 DFT standing in for the in-place FFTW r2c the
   reference driver runs between the two (gen-pk.cpp:193,233)
 
-Pins (tests/test_oracle.py): the known answers of test.cpp:31-86, the
-compiled-reference facts recorded in the survey (corner-mode bins, occupied
-bin counts), brute-force DFT and numpy's pocketfft, closed forms and
+Pins (tests/test_oracle.py): the known answers of test.cpp:31-86, facts of
+the reference compiled from its sources (the bins of the Nyquist corner mode,
+127/255/383/511/1022 at d = 128/256/384/512/1024, and the occupied-bin
+counts), brute-force DFT and numpy's pocketfft, closed forms and
 invariants.  ``bin_of_q`` shares the per-mode log formula of
 ``powerspectrum`` and is pinned by the same facts.
 """
