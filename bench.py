@@ -334,6 +334,14 @@ def main():
     adj = None
     if not args.no_backward:
         adj = fwd_bwd(genpk_amd, torch, pos, args.box, d, nbins, args.backward_steps, args.prec)
+        # the ratio's own target size: 2^24 particles (the same clustered
+        # recipe, 64x fewer clumps) on a 256^3 grid in a 250 Mpc/h box
+        n3, d3, box3 = 1 << 24, 256, 250.0
+        pos3 = synth.clumped_torch(n3, box3, d3, n_clumps=100000 >> 6, seed=5, device=dev)
+        adj["at_256"] = fwd_bwd(genpk_amd, torch, pos3, box3, d3, d3, 10, args.prec)
+        adj["at_256"]["workload"] = f"clustered {n3} particles, d={d3}, box {box3}, nrbins {d3}"
+        del pos3
+        torch.cuda.empty_cache()
 
     bat = None
     if not args.no_batched:

@@ -38,6 +38,13 @@ __device__ __forceinline__ unsigned long long to_fixed(double a, double b, doubl
   return (unsigned long long)__double_as_longlong(t) - 0x4330000000000000ull;
 }
 
+// round(a * b) as a two's complement u64 for |a * b| < 2^51: a * b + 1.5 * 2^52
+// lies in [2^52, 2^53), where the double's ulp is 1 (one rounding, by the FMA)
+__device__ __forceinline__ unsigned long long to_fixed_signed(double a, double b) {
+  const double t = fma(a, b, 6755399441055744.0);
+  return (unsigned long long)__double_as_longlong(t) - 0x4338000000000000ull;
+}
+
 struct DepParams {
   KeyGeom kg;
   int HX, HY, HZ;
@@ -47,13 +54,21 @@ struct DepParams {
   int64_t max_items;   // entries of the item-to-tile table
   double mass;         // constant mass (u64 path)
   double bias;         // 2^52 (to_fixed)
+  const uint32_t *mrange;  // per-particle masses: the sort's |mass| range (float bits)
   int store;           // 1: write tiles, 0: add to tiles
 };
 
-// Records are RW floats: x, y, z, then the mass when NM (FP64 LDS
-// accumulation).  STD: tiles of 16 x 32 x 32 cells (every grid of d >= 32),
-// whose strides are constants, so the 8 corner offsets fold into the LDS
-// instructions' immediate offsets.
+// Records are RW floats: x, y, z, then the mass when NM.  Without masses the
+// weights accumulate in u64 fixed point with 2^36 per unit weight (times the
+// constant mass at the flush).  With masses they accumulate as signed fixed
+// point with 2^s per unit mass, s = 45 - e where every |mass| < 2^e: an item's
+// <= 2^17 terms per cell stay below 2^62 and each term below 2^45.  The
+// absolute error per term, 2^(e-46), stays below the constant-mass path's
+// 2^-37 of a particle's mass when the masses span at most 2^10 (exponents of
+// the largest and smallest nonzero |mass|); wider ranges and non-finite
+// masses accumulate in FP64 LDS atomics instead.  STD: tiles of 16 x 32 x 32
+// cells (every grid of d >= 32), whose strides are constants, so the 8 corner
+// offsets fold into the LDS instructions' immediate offsets.
 template <typename Real, int NM, int RW, bool STD>
 __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict__ rec,
                                                          const uint32_t *__restrict__ tile_start,
@@ -61,7 +76,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
                                                          const uint32_t *__restrict__ item_tile,
                                                          const uint32_t *__restrict__ part_start, DepParams P,
                                                          Real *__restrict__ tiles, double *__restrict__ partials) {
-  using Acc = typename std::conditional<NM == 1, double, unsigned long long>::type;
+  using Acc = unsigned long long;
   constexpr int UNR = 4;
   constexpr int BATCH = DEP_THREADS * UNR;
   extern __shared__ unsigned long long lds_raw[];
@@ -69,6 +84,22 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
   const int TX = STD ? 16 : P.kg.TX, TY = STD ? 32 : P.kg.TY, TZ = STD ? 32 : P.kg.TZ;
   const int sy = STD ? 33 : P.HZ, sx = STD ? 33 * 33 : P.HY * P.HZ;
   const double scale = (NM == 1) ? 1.0 : P.mass * FIXED_INV;
+  bool fixed = true;            // masses: fixed point (else FP64 atomics)
+  double m2s = 1.0, m2inv = 1.0;  // masses: 2^s, 2^-s
+  if constexpr (NM == 1) {
+    const uint32_t mxb = P.mrange[0], mnb = P.mrange[1];
+    fixed = mxb < 0x7f800000u && (mxb == 0u || (int)(mxb >> 23) - (int)(mnb >> 23) <= 10);
+    const int sc = 45 - ((int)(mxb >> 23) - 126);  // every |mass| < 2^((mxb >> 23) - 126)
+    m2s = ldexp(1.0, sc);
+    m2inv = ldexp(1.0, -sc);
+  }
+  // a cell's value at the flush
+  auto cell = [&](int64_t c) -> double {
+    if constexpr (NM == 1)
+      return fixed ? (double)(long long)acc[c] * m2inv : __longlong_as_double((long long)acc[c]);
+    else
+      return (double)acc[c] * scale;
+  };
   float r[UNR][RW], nx[UNR][RW];
   auto load_batch = [&](uint32_t i0, uint32_t pe) {
 #pragma unroll
@@ -123,8 +154,13 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
           const int at = base + off[c];
           if constexpr (NM == 1) {
             const double mw = (double)r[u][3] * wxy[c];
-            atomicAdd(&acc[at], mw * gz);
-            atomicAdd(&acc[at + 1], mw * fz);
+            if (fixed) {
+              atomicAdd(&acc[at], to_fixed_signed(mw, gz * m2s));
+              atomicAdd(&acc[at + 1], to_fixed_signed(mw, fz * m2s));
+            } else {
+              atomicAdd(reinterpret_cast<double *>(&acc[at]), mw * gz);
+              atomicAdd(reinterpret_cast<double *>(&acc[at + 1]), mw * fz);
+            }
           } else {
             atomicAdd(&acc[at], to_fixed(wxy[c], gzs, P.bias));
             atomicAdd(&acc[at + 1], to_fixed(wxy[c], fzs, P.bias));
@@ -146,20 +182,20 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
       if (P.store) {
         for (int64_t c = threadIdx.x; c < P.cm; c += DEP_THREADS) {
           const int64_t a2 = (c >> sz) * P.HZ + (c & (TZ - 1));
-          rows[c] = (Real)((double)acc[a2] * scale);
+          rows[c] = (Real)cell(a2);
         }
-        for (int64_t c = threadIdx.x; c < P.cz; c += DEP_THREADS) zh[c] = (Real)((double)acc[c * P.HZ + TZ] * scale);
+        for (int64_t c = threadIdx.x; c < P.cz; c += DEP_THREADS) zh[c] = (Real)cell(c * P.HZ + TZ);
       } else {
         for (int64_t c = threadIdx.x; c < P.cm; c += DEP_THREADS) {
           const int64_t a2 = (c >> sz) * P.HZ + (c & (TZ - 1));
-          rows[c] = (Real)((double)rows[c] + (double)acc[a2] * scale);
+          rows[c] = (Real)((double)rows[c] + cell(a2));
         }
         for (int64_t c = threadIdx.x; c < P.cz; c += DEP_THREADS)
-          zh[c] = (Real)((double)zh[c] + (double)acc[c * P.HZ + TZ] * scale);
+          zh[c] = (Real)((double)zh[c] + cell(c * P.HZ + TZ));
       }
     } else {
       double *out = partials + (int64_t)(part_start[k] + j) * P.cells;
-      for (int64_t c = threadIdx.x; c < P.cells; c += DEP_THREADS) out[c] = (double)acc[c] * scale;
+      for (int64_t c = threadIdx.x; c < P.cells; c += DEP_THREADS) out[c] = cell(c);
     }
     __syncthreads();  // the flush has read the tile before it is cleared again
     ir = nir;
@@ -290,6 +326,7 @@ void deposit_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass) {
   P.max_items = max_items;
   P.mass = mass;
   P.bias = 4503599627370496.0;
+  P.mrange = o.mrange;
   P.store = store;
   const size_t lds = (size_t)tg.cells * 8;
   {

@@ -94,6 +94,9 @@ struct TileOrder {
   const uint32_t *tile_start = nullptr, *item_start = nullptr, *part_start = nullptr;
   const uint32_t *item_tile = nullptr;  // tile of each work item (max_items entries)
   const uint32_t *nheavy = nullptr, *heavy = nullptr;
+  // with masses: [0] the bits of the largest |mass|, [1] those of the
+  // smallest nonzero |mass| (0xffffffff when every mass is 0)
+  const uint32_t *mrange = nullptr;
   int64_t ks = 0, n = 0, max_items = 0;
 };
 
@@ -152,6 +155,7 @@ struct Plan {
   DevBuf recA, recB, table1, table2, seginfo, tile_start, item_start, part_start, heavy, partials, item_tile;
   DevBuf permA, permB;      // sort permutations kept for the adjoint gather
   DevBuf dig2;              // low tile-key digit of each pass-1 output record (one byte)
+  DevBuf mrange;            // largest and smallest nonzero |mass| of the last sort (float bits)
   DevBuf scan_tmp;
   RetainedOrder order;
 

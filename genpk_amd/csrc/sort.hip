@@ -319,7 +319,8 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
                                                                KeyGeom kg, DigitSel ds, int nb, int hw,
                                                                const uint32_t *__restrict__ offsets,
                                                                uint32_t *__restrict__ perm,
-                                                               uint8_t *__restrict__ dig2, DigitSel ds2) {
+                                                               uint8_t *__restrict__ dig2, DigitSel ds2,
+                                                               uint32_t *__restrict__ mrange) {
   constexpr int RW = Src<NM>::RW;
   constexpr int SUBN = SORT_THREADS * PL;
   constexpr int NBUF = DB ? 2 : 1;
@@ -331,6 +332,7 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
   uint16_t *sdig = reinterpret_cast<uint16_t *>(dbase + nb);  // [SUBN]
   float *stage = reinterpret_cast<float *>(sdig + SUBN);      // [SUBN][RW]
   __shared__ uint32_t wtot[2][SORT_WAVES];
+  __shared__ uint32_t smr[2];  // masses: largest, smallest nonzero |mass| bits of this chunk
 
   int64_t cb, ce, tb, ts;
   if (!cm.range(blockIdx.x, cb, ce, tb, ts)) return;
@@ -340,6 +342,11 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
   for (int i = threadIdx.x; i < nb; i += SORT_THREADS) gbase[i] = offsets[tb + (int64_t)i * ts];
   for (int i = threadIdx.x; i < NBUF * hw * nb; i += SORT_THREADS) whist_all[i] = 0;
   const int row = (wave % hw) * nb;
+  if (threadIdx.x == 0) {
+    smr[0] = 0u;
+    smr[1] = 0xffffffffu;
+  }
+  uint32_t mx = 0u, mn = 0xffffffffu;
 
   float rec[PL][RW];
   const int wbase = wave * (PL * 64) + lane;
@@ -381,6 +388,12 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
 #pragma unroll
         for (int k = 0; k < RW; k++) stage[p * RW + k] = rec[it][k];
         sdig[p] = (uint16_t)(two ? dg | lo[it] << 8 : dg);
+        if constexpr (NM == 1) {
+          // |mass| as bits orders like the value (a NaN above every finite one)
+          const uint32_t mb = __float_as_uint(rec[it][3]) & 0x7fffffffu;
+          mx = max(mx, mb);
+          if (mb) mn = min(mn, mb);
+        }
         if (perm) perm[s0 + loc] = dbase[dg] + p;  // where this input record goes (coalesced)
       }
     }
@@ -413,6 +426,22 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
     for (int i = threadIdx.x; i < hw * nb; i += SORT_THREADS) whist[i] = 0;
     if (!DB) __syncthreads();
   }
+  if constexpr (NM == 1) {
+    if (mrange) {
+      atomicMax(&smr[0], mx);
+      atomicMin(&smr[1], mn);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        atomicMax(&mrange[0], smr[0]);
+        atomicMin(&mrange[1], smr[1]);
+      }
+    }
+  }
+}
+
+__global__ void k_mrange_init(uint32_t *mrange) {
+  mrange[0] = 0u;
+  mrange[1] = 0xffffffffu;
 }
 
 // Pass-2 segment bookkeeping from the scanned pass-1 table.
@@ -518,7 +547,8 @@ DsShape downsweep_shape(int nb, int rw) {
 template <int NM>
 void launch_downsweep(int64_t grid, hipStream_t s, Src<NM> src, float *dst, const ChunkMap &cm, const KeyGeom &kg,
                       const DigitSel &ds, int nb, const uint32_t *offsets, uint32_t *perm,
-                      uint8_t *dig2 = nullptr, const DigitSel &ds2 = DigitSel{0, 0, 0}) {
+                      uint8_t *dig2 = nullptr, const DigitSel &ds2 = DigitSel{0, 0, 0},
+                      uint32_t *mrange = nullptr) {
   GENPK_REQUIRE(nb <= 2 * SORT_THREADS, "sort digit too wide");
   GENPK_REQUIRE(!dig2 || (nb <= 256 && ds2.mask <= 255u), "sort digit bytes need digits of at most 8 bits");
   const DsShape h = downsweep_shape(nb, Src<NM>::RW);
@@ -526,22 +556,22 @@ void launch_downsweep(int64_t grid, hipStream_t s, Src<NM> src, float *dst, cons
   const dim3 g((unsigned)grid), b(SORT_THREADS);
   if (h.pl == 8 && h.db)
     hipLaunchKernelGGL((k_downsweep<NM, 8, true>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
-                       dig2, ds2);
+                       dig2, ds2, mrange);
   else if (h.pl == 8)
     hipLaunchKernelGGL((k_downsweep<NM, 8, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
-                       dig2, ds2);
+                       dig2, ds2, mrange);
   else if (h.pl == 4 && h.db)
     hipLaunchKernelGGL((k_downsweep<NM, 4, true>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
-                       dig2, ds2);
+                       dig2, ds2, mrange);
   else if (h.pl == 4)
     hipLaunchKernelGGL((k_downsweep<NM, 4, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
-                       dig2, ds2);
+                       dig2, ds2, mrange);
   else if (h.pl == 2)
     hipLaunchKernelGGL((k_downsweep<NM, 2, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
-                       dig2, ds2);
+                       dig2, ds2, mrange);
   else
     hipLaunchKernelGGL((k_downsweep<NM, 1, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
-                       dig2, ds2);
+                       dig2, ds2, mrange);
   GENPK_LAUNCH_CHECK();
 }
 
@@ -560,6 +590,13 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
   const float *sorted = nullptr;
   uint32_t *tile_start = nullptr;
   int64_t ks = 0;
+  // with masses, pass 1 (which reads every mass) records their range for the deposit
+  uint32_t *mrange = nullptr;
+  if (NM == 1) {
+    mrange = static_cast<uint32_t *>(p.mrange.ensure(64));
+    hipLaunchKernelGGL(k_mrange_init, dim3(1), dim3(1), 0, s, mrange);
+    GENPK_LAUNCH_CHECK();
+  }
 
   if (tg.passes == 1) {
     const int nb = 1 << tg.hi_bits;
@@ -575,7 +612,7 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     exclusive_scan_u32(t1, t1, (int64_t)nb * g1, p.scan_tmp, s);
     {
       Stage st(p, "sort_downsweep1");
-      launch_downsweep<NM>(g1, s, raw, A, cm, kg, dsel, nb, t1, perm1);
+      launch_downsweep<NM>(g1, s, raw, A, cm, kg, dsel, nb, t1, perm1, nullptr, DigitSel{0, 0, 0}, mrange);
     }
     tile_start = static_cast<uint32_t *>(p.tile_start.ensure((size_t)(ks + 1) * 4));
     hipLaunchKernelGGL(k_tile_start_1pass, dim3((unsigned)ceil_div(ks + 1, 256)), dim3(256), 0, s, t1, g1, (int)ks,
@@ -604,7 +641,7 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     exclusive_scan_u32(t1, t1, (int64_t)nhi * g1, p.scan_tmp, s);
     {
       Stage st(p, "sort_downsweep1");
-      launch_downsweep<NM>(g1, s, raw, A, cm1, kg, d1, nhi, t1, perm1, dig2, d2);
+      launch_downsweep<NM>(g1, s, raw, A, cm1, kg, d1, nhi, t1, perm1, dig2, d2, mrange);
     }
     // segments of pass 2
     uint32_t *seg = static_cast<uint32_t *>(p.seginfo.ensure((size_t)(nhi + 1) * 3 * 4));
@@ -671,6 +708,7 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
   o.part_start = part_start;
   o.nheavy = nheavy;
   o.heavy = heavy;
+  o.mrange = mrange;
   o.ks = ks;
   o.n = n;
   o.max_items = max_items;

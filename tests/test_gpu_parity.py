@@ -124,6 +124,58 @@ def test_deposit_vs_oracle(gpu, d, n, with_masses, prec):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("kind", ["narrow", "wide", "nan", "zero"])
+def test_deposit_mass_ranges_vs_oracle(gpu, kind, prec):
+    """Per-particle masses whose magnitudes span at most 2^10 accumulate in
+    fixed point, wider ranges and non-finite masses in FP64; both match the
+    oracle (a NaN mass makes its eight cells NaN there too)."""
+    import torch
+    announce("deposit masses", kind, prec)
+    d, box, n = 64, 40.0, 200000
+    rng = np.random.default_rng({"narrow": 1, "wide": 2, "nan": 3, "zero": 4}[kind])
+    pos = (rng.random((n, 3)) * box).astype(np.float32)
+    if kind == "narrow":
+        masses = (rng.random(n) * 900 + 1.5).astype(np.float32) * np.where(rng.random(n) < 0.1, -1, 1).astype(np.float32)
+    elif kind == "wide":
+        masses = np.exp(rng.uniform(np.log(1e-5), np.log(10.0), n)).astype(np.float32)
+    elif kind == "nan":
+        masses = (rng.random(n) + 0.5).astype(np.float32)
+        masses[n // 3] = np.nan
+    else:
+        masses = np.zeros(n, dtype=np.float32)
+        masses[::1000] = 2.0
+    got, ref = deposit_case(torch, d, box, pos, masses, 1.0, prec)
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    ok = ~np.isnan(ref)
+    scale = np.abs(ref[ok]).max()
+    tol = 1e-9 if prec == "f64" else 2e-7
+    assert np.abs(got[ok] - ref[ok]).max() <= tol * scale
+    if kind == "wide":
+        # the light particles' cells keep their relative accuracy
+        light = ok & (np.abs(ref) > 0) & (np.abs(ref) < 1e-3)
+        assert light.any()
+        rel = np.abs(got[light] - ref[light]) / np.abs(ref[light])
+        assert rel.max() <= (1e-9 if prec == "f64" else 1e-6)
+
+
+def test_deposit_masses_fixed_point_is_deterministic(gpu):
+    """Masses within the fixed-point range give bit-identical grids on
+    repeated deposits (integer sums do not depend on the atomics' order)."""
+    import torch
+    announce("deposit masses determinism")
+    d, box, n = 64, 64.0, 400000
+    rng = np.random.default_rng(9)
+    pos = np.concatenate([rng.normal(20.3, 0.4, (300000, 3)), rng.random((n - 300000, 3)) * box]).astype(np.float32)
+    masses = (rng.random(n) + 0.5).astype(np.float32)
+    grids = []
+    for _ in range(3):
+        p = genpk_amd.Plan(d, "f64")
+        p.deposit(to_dev(torch, pos), box, to_dev(torch, masses), 1.0)
+        grids.append(p.export_density().cpu().numpy())
+    assert np.array_equal(grids[0], grids[1]) and np.array_equal(grids[0], grids[2])
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("with_masses", [False, True])
 def test_deposit_wide_key_vs_oracle(gpu, with_masses, prec):
     """d = 1280: 80 x 40 x 40 = 128000 tiles, a 17-bit key whose 9-bit low
