@@ -175,14 +175,38 @@ __global__ void __launch_bounds__(FFT_THREADS) k_c2r_z(T *__restrict__ field, in
   const int W = M + 1;
   const int64_t row0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * L;
   const int nl = (int)min((int64_t)L, nrows - row0);
-  for (int i = threadIdx.x; i < L * W; i += FFT_THREADS) {
-    const int l = (int)fdM1.div((uint32_t)i), k = i - l * W;
-    C v = C{0, 0};
-    if (l < nl) {
-      const T *row = field + (row0 + l) * rowlen;
-      v = C{row[2 * k], (k == 0 || k == M) ? (T)0 : row[2 * k + 1]};
+  // one row element is one thread's column (M == FFT_THREADS, L <= MAXE
+  // rows): element k = threadIdx.x of every row as one 8- or 16-byte load,
+  // k = M by thread 0; no index division
+  const bool colwise = M == FFT_THREADS && L <= MAXE;
+  if (colwise) {
+    const int k = threadIdx.x;
+    C v[MAXE];
+#pragma unroll
+    for (int l = 0; l < MAXE; l++) {
+      v[l] = C{0, 0};
+      if (l < nl) v[l] = reinterpret_cast<const C *>(field + (row0 + l) * rowlen)[k];
     }
-    buf[i] = v;
+#pragma unroll
+    for (int l = 0; l < MAXE; l++) {
+      if (l < L) {
+        if (k == 0) v[l].y = (T)0;
+        buf[l * W + k] = v[l];
+      }
+    }
+    if (k == 0)
+      for (int l = 0; l < L; l++)
+        buf[l * W + M] = l < nl ? C{field[(row0 + l) * rowlen + 2 * M], (T)0} : C{0, 0};
+  } else {
+    for (int i = threadIdx.x; i < L * W; i += FFT_THREADS) {
+      const int l = (int)fdM1.div((uint32_t)i), k = i - l * W;
+      C v = C{0, 0};
+      if (l < nl) {
+        const T *row = field + (row0 + l) * rowlen;
+        v = C{row[2 * k], (k == 0 || k == M) ? (T)0 : row[2 * k + 1]};
+      }
+      buf[i] = v;
+    }
   }
   __syncthreads();
   // Z_k = (X_k + conj X_{M-k}) + i w^-k (X_k - conj X_{M-k}), k < M; staged in registers
@@ -191,7 +215,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_c2r_z(T *__restrict__ field, in
   for (int q = 0; q < MAXE; q++) {
     const int i = threadIdx.x + q * FFT_THREADS;
     if (i < L * M) {
-      const int l = (int)fdM.div((uint32_t)i), k = i - l * M;
+      const int l = colwise ? q : (int)fdM.div((uint32_t)i), k = colwise ? (int)threadIdx.x : i - l * M;
       const C xk = buf[l * W + k];
       const C xc = cconj(buf[l * W + (M - k)]);
       const C s = cadd(xk, xc);
@@ -212,6 +236,18 @@ __global__ void __launch_bounds__(FFT_THREADS) k_c2r_z(T *__restrict__ field, in
   for (int i = threadIdx.x; i < M; i += FFT_THREADS) stw[i] = twM[i];
   __syncthreads();
   lds_fft<+1, false, MAXE, C, true>(buf, rp, L, 0, 1, M, stw);
+  if (colwise) {
+    const int m = threadIdx.x;
+#pragma unroll
+    for (int l = 0; l < MAXE; l++) {
+      if (l < nl) {
+        C *row = reinterpret_cast<C *>(field + (row0 + l) * rowlen);
+        row[m] = buf[lds_ix<true>(l * M + m)];
+        if (m == 0) row[M] = C{0, 0};
+      }
+    }
+    return;
+  }
   for (int i = threadIdx.x; i < nl * M; i += FFT_THREADS) {
     const int l = (int)fdM.div((uint32_t)i), m = i - l * M;
     const C v = buf[lds_ix<true>(i)];

@@ -92,6 +92,45 @@ def test_r2c_c2r_plan(gpu, d, prec, tol):
     assert np.abs(back - x).max() <= 10 * tol * np.abs(x).max()
 
 
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 2e-5)])
+def test_r2c_c2r_plan_1024(gpu, prec, tol):
+    """d = 1024 (a z row is one element per thread in both z passes), checked
+    on the GPU: a plane wave's single spike, Parseval on a random field, and
+    c2r(r2c(x)) = d^3 x."""
+    import torch
+    announce("r2c_c2r 1024", prec)
+    d = 1024
+    dt = torch.float32 if prec == "f32" else torch.float64
+    p = genpk_amd.Plan(d, prec)
+    grid = torch.zeros((d, d, 2 * (d // 2 + 1)), dtype=dt, device="cuda")
+    ax = torch.arange(d, device="cuda", dtype=torch.float64) * (2 * np.pi / d)
+    k0 = (3, 5, 7)
+    grid[:, :, :d] = torch.cos(k0[0] * ax[:, None, None] + k0[1] * ax[None, :, None] + k0[2] * ax[None, None, :]).to(dt)
+    p.import_density(grid)
+    F = p.r2c()
+    half = d ** 3 / 2
+    spike = complex(F[k0].cpu())
+    assert abs(spike - half) <= tol * half
+    F[k0] = 0
+    assert float(F.abs().max().cpu()) <= tol * half
+    del F
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    x = torch.randn((d, d, d), generator=g, device="cuda", dtype=dt)
+    grid[:, :, :d] = x
+    p.import_density(grid)
+    F = p.r2c()
+    h = torch.full((d // 2 + 1,), 2.0, dtype=torch.float64, device="cuda")
+    h[0] = h[d // 2] = 1.0
+    lhs = float(x.double().pow(2).sum().cpu())
+    rhs = float(((F.abs().double() ** 2).sum(dim=(0, 1)) * h).sum().cpu()) / d ** 3
+    assert abs(rhs - lhs) <= tol * lhs
+    del F
+    back = p.c2r()[:, :, :d]
+    err = float((back.double() / d ** 3 - x.double()).abs().max().cpu())
+    assert err <= 10 * tol * float(x.abs().max().cpu())
+
+
 # ----------------------------------------------------------------- deposit
 
 def deposit_case(torch, d, box, pos, masses, mass, prec):
