@@ -101,7 +101,7 @@ def tile_geometry(d):
     return K, (tx + 1) * (ty + 1) * (tz + 1), tx * ty * tz
 
 
-def fwd_bwd(genpk_amd, torch, pos, box, d, nbins, steps, prec, warm=1):
+def fwd_bwd(genpk_amd, torch, pos, box, d, nbins, steps, prec, warm=1, reps=1):
     """Forward + backward of the band-power chi^2 loss (target: the forward
     P times 1.05) with the recompute policy and a masses tensor (all ones)
     that receives gradients, against the forward on the same inputs."""
@@ -126,18 +126,23 @@ def fwd_bwd(genpk_amd, torch, pos, box, d, nbins, steps, prec, warm=1):
         return torch.autograd.grad(chi2_loss(P, target, cnt), (tp, tm))
 
     def timed(f, *a):
+        # the median over `reps` groups of `steps` calls (a one-off stall of
+        # the device or the host does not move it)
         for _ in range(warm):
             f(*a)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            f(*a)
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t0) * 1e3 / steps
+        groups = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                f(*a)
+            torch.cuda.synchronize()
+            groups.append((time.perf_counter() - t0) * 1e3 / steps)
+        return sorted(groups)[len(groups) // 2]
 
     t_f = timed(fwd)
     out = {"loss": "sum_b (P_b - T_b)^2 / sigma_b^2, sigma_b = T_b sqrt(2/count_b), T = 1.05 P",
-           "grads": "positions and masses", "steps": steps, "ms_fwd_same_inputs": round(t_f, 3)}
+           "grads": "positions and masses", "steps": steps, "reps": reps, "ms_fwd_same_inputs": round(t_f, 3)}
     for policy in ("always", "auto"):
         t = timed(fb, policy)
         out[f"ms_fwd_bwd_{policy}"] = round(t, 3)
@@ -250,6 +255,18 @@ def main():
     n = 1 << args.n_log2
     d = args.dims
     nbins = d
+    at_256 = None
+    if not args.no_backward:
+        # the fwd+bwd ratio at its own target size, measured before anything
+        # large is allocated: 2^24 particles (the same clustered recipe, 64x
+        # fewer clumps) on a 256^3 grid in a 250 Mpc/h box; steps of a few
+        # ms, so the median of 5 groups of 10 after 3 warm-up calls
+        n3, d3, box3 = 1 << 24, 256, 250.0
+        pos3 = synth.clumped_torch(n3, box3, d3, n_clumps=100000 >> 6, seed=5, device=dev)
+        at_256 = fwd_bwd(genpk_amd, torch, pos3, box3, d3, d3, 10, args.prec, warm=3, reps=5)
+        at_256["workload"] = f"clustered {n3} particles, d={d3}, box {box3}, nrbins {d3}"
+        del pos3
+        torch.cuda.empty_cache()
     t_gen = time.perf_counter()
     pos = synth.clumped_torch(n, args.box, d, n_clumps=100000, seed=4, device=dev)
     torch.cuda.synchronize()
@@ -301,6 +318,13 @@ def main():
     b_fwd = 12 * n + 4 * c * p_pad
     pipeline_gbs = b_fwd / (ms_step * 1e-3) / 1e9
 
+    # forward + backward before the FP64 check and the calibration free
+    # their large buffers (timed work right after large frees runs slower)
+    adj = None
+    if not args.no_backward:
+        adj = fwd_bwd(genpk_amd, torch, pos, args.box, d, nbins, args.backward_steps, args.prec)
+        adj["at_256"] = at_256
+
     check = None
     if not args.no_check:
         pw_t = pw.double().cpu().numpy()
@@ -332,19 +356,6 @@ def main():
         torch.cuda.synchronize()
         del a, b, out
 
-    adj = None
-    if not args.no_backward:
-        # the ratio's own target size first: 2^24 particles (the same clustered
-        # recipe, 64x fewer clumps) on a 256^3 grid in a 250 Mpc/h box; its
-        # steps are short (a few ms), so more of them after three warm-up calls
-        n3, d3, box3 = 1 << 24, 256, 250.0
-        pos3 = synth.clumped_torch(n3, box3, d3, n_clumps=100000 >> 6, seed=5, device=dev)
-        at_256 = fwd_bwd(genpk_amd, torch, pos3, box3, d3, d3, 30, args.prec, warm=3)
-        at_256["workload"] = f"clustered {n3} particles, d={d3}, box {box3}, nrbins {d3}"
-        del pos3
-        torch.cuda.empty_cache()
-        adj = fwd_bwd(genpk_amd, torch, pos, args.box, d, nbins, args.backward_steps, args.prec)
-        adj["at_256"] = at_256
 
     bat = None
     if not args.no_batched:
