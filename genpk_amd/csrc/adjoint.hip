@@ -74,8 +74,11 @@ __global__ void k_weight(typename Cx<T>::C *__restrict__ f, int64_t d, int64_t h
 
 // x pass of the adjoint on the field after the forward z and y passes:
 // per column block (fixed y, B consecutive kz), forward length-d FFT along
-// x, multiply by the adjoint weight, inverse FFT, store back.
-template <typename T>
+// x, multiply by the adjoint weight, inverse FFT, store back.  PP (n * B =
+// ADJ_THREADS * MAXE and a first stage of radix MAXE): the first stage runs
+// from the loaded registers and the stages alternate between two LDS
+// buffers (one barrier each).
+template <typename T, bool PP>
 __global__ void __launch_bounds__(ADJ_THREADS) k_adjoint_x(typename Cx<T>::C *__restrict__ f, int64_t d, int64_t hz,
                                                           int lgb, int nkb, RadixPlan rp,
                                                           const typename Cx<T>::C *__restrict__ tw, AdjWeight w) {
@@ -100,10 +103,37 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_adjoint_x(typename Cx<T>::C *__
     const int e = i >> lgb, b = i & (B - 1);
     v[q] = (i < tot && b < nb) ? base[e * estride + b] : C{0, 0};
   }
+  if constexpr (!PP) {
 #pragma unroll
-  for (int q = 0; q < MAXE; q++) {
-    const int i = threadIdx.x + q * ADJ_THREADS;
-    if (i < tot) buf[i] = v[q];
+    for (int q = 0; q < MAXE; q++) {
+      const int i = threadIdx.x + q * ADJ_THREADS;
+      if (i < tot) buf[i] = v[q];
+    }
+  }
+  if constexpr (PP) {
+    C *buf2 = buf + tot;
+    C *stw = buf2 + tot;
+    for (int i = threadIdx.x; i < n; i += ADJ_THREADS) stw[i] = tw[i];
+    fft_first_regs<MAXE, -1>(v, buf2, B, lgb, B, 1);
+    __syncthreads();
+    C *res = lds_fft_pp<-1, true, MAXE>(buf2, buf, rp, B, lgb, B, 1, stw, 1);
+    for (int i = threadIdx.x; i < tot; i += ADJ_THREADS) {
+      const int e = i >> lgb, b = i & (B - 1);
+      if (b < nb) {
+        const double s = w.at(e, y, kz0 + b, d);
+        const C a = res[i];
+        res[i] = C{(T)((double)a.x * s), (T)((double)a.y * s)};
+      }
+    }
+    __syncthreads();
+    res = lds_fft_pp<+1, true, MAXE>(res, res == buf ? buf2 : buf, rp, B, lgb, B, 1, stw);
+#pragma unroll
+    for (int q = 0; q < MAXE; q++) {
+      const int i = threadIdx.x + q * ADJ_THREADS;
+      const int e = i >> lgb, b = i & (B - 1);
+      if (i < tot && b < nb) base[e * estride + b] = res[i];
+    }
+    return;
   }
   C *stw = buf + tot;
   for (int i = threadIdx.x; i < n; i += ADJ_THREADS) stw[i] = tw[i];
@@ -250,8 +280,16 @@ void adjoint_x_impl(Plan &p, const AdjWeight &w) {
   const C *tw = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
   GENPK_REQUIRE((int64_t)p.d * B <= (int64_t)AdjCfg<T>::MAXE * ADJ_THREADS, "adjoint x pass: column block too large");
   Stage st(p, "adjoint_x");
-  hipLaunchKernelGGL(k_adjoint_x<T>, dim3((unsigned)(p.d * nkb)), dim3(ADJ_THREADS), (size_t)p.d * (B + 1) * sizeof(C),
-                     p.stream, p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, rp, tw, w);
+  const int64_t tot = p.d * B;
+  const bool pp = rp.st[0].R == AdjCfg<T>::MAXE && tot == (int64_t)ADJ_THREADS * AdjCfg<T>::MAXE &&
+                  (size_t)(2 * tot + p.d) * sizeof(C) <= 160 * 1024;
+  const dim3 grid((unsigned)(p.d * nkb));
+  if (pp)
+    hipLaunchKernelGGL((k_adjoint_x<T, true>), grid, dim3(ADJ_THREADS), (size_t)(2 * tot + p.d) * sizeof(C), p.stream,
+                       p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, rp, tw, w);
+  else
+    hipLaunchKernelGGL((k_adjoint_x<T, false>), grid, dim3(ADJ_THREADS), (size_t)(tot + p.d) * sizeof(C), p.stream,
+                       p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, rp, tw, w);
   GENPK_LAUNCH_CHECK();
 }
 

@@ -114,10 +114,10 @@ def test_adjoint_scalar_mass(gpu):
 
 def test_adjoint_policies_agree(gpu):
     """The three recompute policies give the same gradients to rounding
-    (1e-12 for P, 1e-10 of the largest gradient component): per-particle
-    masses are accumulated with FP64 LDS atomics, whose order varies from
-    run to run, and the "never" forward bins the stored spectrum while the
-    others use the fused x-pass binning, whose sums run in another order."""
+    (1e-12 for P, 1e-10 of the largest gradient component): the bins' FP64
+    LDS atomics run in a varying order, and the "never" forward bins the
+    stored spectrum while the others use the fused x-pass binning, whose
+    sums run in another order."""
     import torch
     d, box, n_side, nrbins = 48, 100.0, 32, 48
     announce("adjoint_policies", f"d={d} n={n_side}^3")
@@ -131,6 +131,42 @@ def test_adjoint_policies_agree(gpu):
     assert _close(r1[2], r2[2], 1e-10)[0] and _close(r1[3], r2[3], 1e-10)[0]
     assert np.allclose(r2[0], r3[0], rtol=1e-12, atol=0)
     assert _close(r2[2], r3[2], 1e-10)[0] and _close(r2[3], r3[3], 1e-10)[0]
+
+
+def test_adjoint_x_1024_matches_unfused(gpu):
+    """d = 1024, FP32, where the adjoint x pass runs its first stage from the
+    loaded registers and alternates two LDS buffers: gradients with
+    recompute "always" (fused forward FFT -> weight -> inverse FFT per column
+    block) equal those of "never" (the weight applied to the kept spectrum,
+    then the unfused inverse passes) to FP32 rounding (2e-4 of the largest
+    component)."""
+    import torch
+    from genpk_amd.autograd import chi2_loss
+    d, box, n = 1024, 1000.0, 1 << 21
+    announce("adjoint_x_1024", f"n={n}")
+    pos = synth.clumped(n, box, d, n_clumps=256, seed=21).astype(np.float32)
+    m = (np.random.default_rng(21).random(n) + 0.5).astype(np.float32)
+    tp = torch.from_numpy(pos).cuda()
+    tm = torch.from_numpy(m).cuda()
+    M = float(m.astype(np.float64).sum())
+    plan = genpk_amd.Plan(d, "f32")
+    plan.clear()
+    plan.deposit(tp, box, tm)
+    P0, cnt, _ = plan.power(d, mass=M)
+    target = P0 * 1.05
+
+    def grads(policy):
+        a = tp.clone().requires_grad_(True)
+        b = tm.clone().requires_grad_(True)
+        P = power_spectrum(a, box, d, d, masses=b, total_mass=M, plan=plan, recompute=policy)
+        return [g.double().cpu().numpy() for g in torch.autograd.grad(chi2_loss(P, target, cnt), (a, b))]
+
+    g_never = grads("never")
+    g_always = grads("always")
+    for ga, gn in zip(g_always, g_never):
+        ok, e = _close(ga, gn, 2e-4)
+        print(f"max rel grad diff {e:.3e}", flush=True)
+        assert ok
 
 
 def test_adjoint_plan_flows(gpu):
