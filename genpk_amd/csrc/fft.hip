@@ -36,7 +36,11 @@ template <> struct FftCfg<double> { static constexpr int ELEMS = 4096; static co
 // register budget of two waves per SIMD is available.  The rows sit in LDS
 // with one element of padding per 32 (lds_ix), which spreads the strided
 // writes of the FFT stages over the banks.
-template <typename T, bool FROM_TILES>
+// BFLY (FP32, from the tile store, M == FFT_THREADS, a first stage of radix
+// 8): each thread loads the eight inputs of up to two butterflies of the
+// first stage straight from the tile store and runs that stage from
+// registers.
+template <typename T, bool FROM_TILES, bool BFLY = false>
 __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(1, 2))) k_r2c_z(T *__restrict__ field, const T *__restrict__ tiles,
                                                       FoldGeom fg, FastDiv fdd, int64_t rowlen, int64_t nrows, int L,
                                                       FastDiv fdM, FastDiv fdM1, RadixPlan rp,
@@ -50,6 +54,58 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
   const int64_t row0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * L;
   const int nl = (int)min((int64_t)L, nrows - row0);
   const int tot = L * M;
+  // twiddles of the M-point FFT and of the even/odd split, staged in LDS
+  C *stw = buf + lds_len<true>((int64_t)L * M);  // [M]
+  C *sxw = stw + M;               // [M+1]
+  if constexpr (BFLY) {
+    // butterfly u = threadIdx.x + FFT_THREADS * i: row u / NQ, inputs
+    // u % NQ + NQ * r (NQ = M / 8)
+    constexpr int NQ = FFT_THREADS / 8;
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const int u = threadIdx.x + FFT_THREADS * i;
+      const int b = u / NQ, j = u % NQ;
+      if (b < L) {
+        C w[8];
+        if (b < nl) {
+          const int64_t row = row0 + b;
+          const int x = (int)fdd.div((uint32_t)row), y = (int)(row - (int64_t)x * fg.d);
+          const int64_t txy = (int64_t)(x >> fg.sx) * fg.nty + (y >> fg.sy);
+          const int64_t rr = (int64_t)(x & (fg.TX - 1)) * fg.HY + (y & (fg.TY - 1));
+          T zh8[8];
+#pragma unroll
+          for (int r = 0; r < 8; r++) {
+            const int z = 2 * (j + NQ * r);
+            const int tz = z >> fg.sz, lz = z & (fg.TZ - 1);
+            const int64_t at = txy * fg.ntz * fg.cm + (int64_t)tz * fg.cm + rr * fg.TZ + lz;
+            w[r] = C{tiles[at], tiles[at + 1]};
+            zh8[r] = (T)0;
+            if (lz == 0) {
+              // the previous tile along z's z-halo cell (in its row store when partial)
+              const int tzp = tz == 0 ? fg.ntz - 1 : tz - 1;
+              const int hw = min(fg.TZ, fg.d - tzp * fg.TZ);
+              const int64_t h = hw < fg.TZ ? (int64_t)tzp * fg.cm + hw + txy * fg.ntz * fg.cm + rr * fg.TZ
+                                           : fg.zoff + (int64_t)tzp * fg.cz + txy * fg.ntz * fg.cz + rr;
+              zh8[r] = tiles[h];
+            }
+          }
+#pragma unroll
+          for (int r = 0; r < 8; r++) w[r].x = w[r].x + zh8[r];  // as the element-wise load: one rounding
+        } else {
+#pragma unroll
+          for (int r = 0; r < 8; r++) w[r] = C{0, 0};
+        }
+        Dft<8, -1, C>::run(w);
+#pragma unroll
+        for (int r = 0; r < 8; r++) buf[lds_ix<true>(b * M + 8 * j + r)] = w[r];
+      }
+    }
+    for (int i = threadIdx.x; i < M; i += FFT_THREADS) stw[i] = twM[i];
+    for (int i = threadIdx.x; i <= M; i += FFT_THREADS) sxw[i] = twd[i];
+    __syncthreads();
+    for (int st = 1; st < rp.nst; st++)
+      fft_stage_r<-1, false, MAXE, false, C, true>(rp.st[st].R, buf, buf, L, 0, 1, M, rp.st[st], stw);
+  } else {
   C v[MAXE];
   T zh[FROM_TILES ? MAXE : 1];
   if (FROM_TILES && M == FFT_THREADS) {
@@ -113,13 +169,11 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
     if (FROM_TILES) v[q].x = v[q].x + zh[FROM_TILES ? q : 0];
     if (i < tot) buf[lds_ix<true>(i)] = v[q];
   }
-  // twiddles of the M-point FFT and of the even/odd split, staged in LDS
-  C *stw = buf + lds_len<true>((int64_t)L * M);  // [M]
-  C *sxw = stw + M;               // [M+1]
   for (int i = threadIdx.x; i < M; i += FFT_THREADS) stw[i] = twM[i];
   for (int i = threadIdx.x; i <= M; i += FFT_THREADS) sxw[i] = twd[i];
   __syncthreads();
   lds_fft<-1, false, MAXE, C, true>(buf, rp, L, 0, 1, M, stw);
+  }
   // X_k = (Z_k + conj Z_{M-k})/2 - i/2 * w^k (Z_k - conj Z_{M-k}),  w = exp(-2 pi i / d)
   auto split = [&](C zk, C zc, C w) {
     const C e = C{(zk.x + zc.x) * (T)0.5, (zk.y + zc.y) * (T)0.5};
@@ -364,7 +418,14 @@ static void r2c_z_impl(Plan &p, bool from_tiles) {
   const FoldGeom fg = fold_geom(p);
   const dim3 grid((unsigned)ceil_div(nrows, L));
   const size_t lds = ((size_t)lds_len<true>((int64_t)L * M) + 2 * M + 1) * sizeof(C);
-  if (from_tiles) {
+  const bool bfly = sizeof(T) == 4 && M == FFT_THREADS && rp.st[0].R == 8 && rp.nst >= 2 &&
+                    (int64_t)L * (M / 8) <= 2 * FFT_THREADS;
+  if (from_tiles && bfly) {
+    Stage st(p, "fold_fft_z");
+    hipLaunchKernelGGL((k_r2c_z<T, true, true>), grid, dim3(FFT_THREADS), lds, p.stream, p.field.as<T>(),
+                       p.tiles.as<const T>(), fg, FastDiv((uint32_t)p.d), p.rowlen, nrows, L, FastDiv((uint32_t)M),
+                       FastDiv((uint32_t)(M + 1)), rp, twM, twd);
+  } else if (from_tiles) {
     Stage st(p, "fold_fft_z");
     hipLaunchKernelGGL((k_r2c_z<T, true>), grid, dim3(FFT_THREADS), lds, p.stream, p.field.as<T>(),
                        p.tiles.as<const T>(), fg, FastDiv((uint32_t)p.d), p.rowlen, nrows, L, FastDiv((uint32_t)M),

@@ -422,6 +422,36 @@ def test_counts_1024_reference_facts(gpu):
     assert int(c.sum()) == 1024 ** 3 - 1
 
 
+@pytest.mark.parametrize("with_masses", [False, True])
+def test_power_1024_fp32_vs_fp64(gpu, with_masses):
+    """d = 1024, where the FP32 forward reads a z row one element per thread
+    or a first-stage butterfly per thread straight from the tile store, and
+    bins in the fused x pass: P against the FP64 plan's (element-wise z pass)
+    on the same clustered particles within 2e-5 per occupied bin, and equal
+    counts."""
+    import torch
+    announce("power d=1024 f32 vs f64", with_masses)
+    d, box, n = 1024, 1000.0, 1 << 22
+    pos = synth.clumped_torch(n, box, d, n_clumps=400, seed=31, device="cuda")
+    masses = (torch.rand(n, generator=torch.Generator(device="cuda").manual_seed(31), device="cuda") + 0.5) \
+        if with_masses else None
+    M = float(masses.double().sum()) if with_masses else float(n)
+    out = {}
+    for prec in ("f32", "f64"):
+        p = genpk_amd.Plan(d, prec)
+        p.deposit(pos, box, masses)
+        P, c, _ = p.power(d, mass=M)
+        out[prec] = (P.cpu().numpy(), c.cpu().numpy())
+        del p
+        torch.cuda.empty_cache()
+    (p32, c32), (p64, c64) = out["f32"], out["f64"]
+    assert np.array_equal(c32, c64)
+    occ = c64 > 0
+    rel = np.abs(p32[occ] / p64[occ] - 1)
+    print(f"max rel {rel.max():.3e}", flush=True)
+    assert rel.max() <= 2e-5
+
+
 # ----------------------------------------------------------------- errors
 
 def test_errors_are_reported(gpu):
