@@ -225,6 +225,9 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
       if (NF == 2) vb[NF == 2 ? q : 0] = ok ? fb[off + e * estride + b] : C{0, 0};
     }
   };
+  // REG1: block v + gridDim.x loads into the registers while block v is
+  // binned (their live range misses the FFT stages, where the registers peak)
+  if (REG1 && (int64_t)blockIdx.x < nblk) load(blockIdx.x);
   for (int64_t v = blockIdx.x; v < nblk; v += gridDim.x) {
     const int64_t blk = xcd_block((uint32_t)v, (uint32_t)nblk);
     const int64_t y = blk / nkb;
@@ -232,7 +235,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
     const int64_t kz0 = (int64_t)kb * B;
     const int nb = (int)min((int64_t)B, hz - kz0);
     const int64_t off = y * hz + kz0;
-    load(v);
+    if (!REG1) load(v);
     C *res = buf;
     if constexpr (REG1) {
       fft_first_regs<MAXE, -1>(va, buf2, B, lgb, B, 1);
@@ -265,6 +268,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
         }
       }
     }
+    if (REG1 && v + gridDim.x < nblk) load(v + gridDim.x);
     if (my_b < nb) {
       const int kz = (int)(kz0 + my_b);
       const int ky = (int)kval(y, d);
@@ -285,7 +289,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
           const int e = e0 + j;
           if (j < per && e < n) {
             ma[BATCH ? j : 0] = res[e * B + my_b];
-            wx[BATCH ? j : 0] = wt[e];
+            if (!REG1) wx[BATCH ? j : 0] = wt[e];
           }
         }
       }
@@ -297,7 +301,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
           const int q = a + kx * kx;
           if (q != 0) {
             const int b = bt.of(q, thr32, g.nrbins, g.lq);
-            const float gw = ((BATCH ? wx[BATCH ? j : 0] : wt[e]) * wy) * wz;  // the reference's ((wx*wy)*wz) in float
+            const float gw = ((BATCH && !REG1 ? wx[BATCH ? j : 0] : wt[e]) * wy) * wz;  // the reference's ((wx*wy)*wz) in float
             double dc = (double)gw * (double)gw;
             dc *= dc;
             const C fa_m = BATCH ? ma[BATCH ? j : 0] : res[e * B + my_b];
