@@ -93,12 +93,15 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
     m2s = ldexp(1.0, sc);
     m2inv = ldexp(1.0, -sc);
   }
-  // a cell's value at the flush
+  // a cell's value at the flush, which clears the cell for the next item
+  // (every cell is read once, by one thread)
   auto cell = [&](int64_t c) -> double {
+    const Acc a = acc[c];
+    acc[c] = 0;
     if constexpr (NM == 1)
-      return fixed ? (double)(long long)acc[c] * m2inv : __longlong_as_double((long long)acc[c]);
+      return fixed ? (double)(long long)a * m2inv : __longlong_as_double((long long)a);
     else
-      return (double)acc[c] * scale;
+      return (double)a * scale;
   };
   float r[UNR][RW], nx[UNR][RW];
   auto load_batch = [&](uint32_t i0, uint32_t pe) {
@@ -117,11 +120,12 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
   ItemRange ir;
   bool have = item_range(item_tile, item_start, tile_start, P.ks, P.max_items, item, ir);
   if (have) load_batch(ir.pb, ir.pe);
+  // the tile starts cleared; each flush clears the cells it reads
+  for (int64_t c = threadIdx.x; c < P.cells; c += DEP_THREADS) acc[c] = 0;
+  __syncthreads();
   while (have) {
     const int64_t k = ir.k;
     const uint32_t m = ir.m, j = ir.j, pb = ir.pb, pe = ir.pe;
-    for (int64_t c = threadIdx.x; c < P.cells; c += DEP_THREADS) acc[c] = 0;
-    __syncthreads();
     const int ntz = P.kg.ntz, nty = P.kg.nty;
     const int tz = (int)(k % ntz);
     const int ty = (int)((k / ntz) % nty);
@@ -197,7 +201,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
       double *out = partials + (int64_t)(part_start[k] + j) * P.cells;
       for (int64_t c = threadIdx.x; c < P.cells; c += DEP_THREADS) out[c] = cell(c);
     }
-    __syncthreads();  // the flush has read the tile before it is cleared again
+    __syncthreads();  // the flush has read and cleared the tile before the next item adds to it
     ir = nir;
     have = next;
   }
