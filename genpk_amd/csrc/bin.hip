@@ -175,7 +175,12 @@ struct BinTrack {
 // then runs from registers into buf2.  The previous block no longer reads
 // buf2 (an even number of stages leaves its result in buf), so no barrier
 // comes before that stage.
-template <typename T, int NF, bool REG1>
+//
+// R2L (with REG1, a last stage of radix 2): the last stage runs in the
+// binning, each thread pairing stage inputs j and j + n/2 into modes j and
+// j + n/2 in registers; the two buffers then swap roles between blocks so
+// that the next first stage never writes the buffer being binned.
+template <typename T, int NF, bool REG1, bool R2L = false>
 __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__restrict__ fa,
                                                             typename Cx<T>::C *__restrict__ fb, BinGeom g, int B,
                                                             int nkb, RadixPlan rp,
@@ -183,6 +188,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
                                                             int pp, double *__restrict__ part) {
   using C = typename Cx<T>::C;
   constexpr int NS = NF == 1 ? 1 : 3;
+  static_assert(!R2L || REG1, "the radix-2 last stage in the binning needs the register first stage");
   extern __shared__ unsigned char lds_raw[];
   double *bins = reinterpret_cast<double *>(lds_raw);
   C *buf = reinterpret_cast<C *>(bins + ((NS * g.nrbins + 1) & ~1));  // [NF][n][B], 16-byte aligned
@@ -240,7 +246,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
     if constexpr (REG1) {
       fft_first_regs<MAXE, -1>(va, buf2, B, lgb, B, 1);
       __syncthreads();
-      res = lds_fft_pp<-1, true, MAXE>(buf2, buf, rp, B, lgb, B, 1, stw, 1);
+      res = lds_fft_pp<-1, true, MAXE>(buf2, buf, rp, B, lgb, B, 1, stw, 1, R2L ? rp.nst - 1 : -1);
     } else {
       __syncthreads();  // the previous block's binning is done with buf
 #pragma unroll
@@ -280,6 +286,38 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
       // this thread's run of at most MAXE modes along x (per <= MAXE as
       // n * B <= BIN_THREADS * MAXE); with one field all its LDS reads are
       // issued first (two fields would not fit the registers)
+      if constexpr (R2L) {
+        // the last stage: butterfly jj pairs stage inputs jj and jj + n/2
+        // (the latter times w^jj) into modes jj and jj + n/2
+        constexpr int HB = MAXE / 2;
+        const int half = n / 2;
+        const int jb = (threadIdx.x / B) * HB;
+        C mo[MAXE];
+#pragma unroll
+        for (int q = 0; q < HB; q++) {
+          const int jj = jb + q;
+          const C a0 = res[jj * B + my_b];
+          const C v1 = cmul(res[(jj + half) * B + my_b], stw[jj]);
+          mo[q] = cadd(a0, v1);
+          mo[q + HB] = csub(a0, v1);
+        }
+#pragma unroll
+        for (int q = 0; q < MAXE; q++) {
+          const int e = q < HB ? jb + q : jb + (q - HB) + half;
+          const int kx = e <= d / 2 ? e : e - (int)d;
+          const int qk = a + kx * kx;
+          if (qk != 0) {
+            const int bn = bt.of(qk, thr32, g.nrbins, g.lq);
+            const float gw = (wt[e] * wy) * wz;  // the reference's ((wx*wy)*wz) in float
+            double dc = (double)gw * (double)gw;
+            dc *= dc;
+            double c[3];
+            mode_terms<C, NF>(mo[q], mo[q], h, dc, c);
+            run_add<NS>(r, bn, c, bins, g.nrbins);
+          }
+        }
+        run_flush<NS>(r, bins, g.nrbins);
+      } else {
       constexpr bool BATCH = NF == 1;
       C ma[BATCH ? MAXE : 1];
       float wx[BATCH ? MAXE : 1];
@@ -313,6 +351,15 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
         }
       }
       run_flush<NS>(r, bins, g.nrbins);
+      }
+    }
+    if constexpr (R2L) {
+      // the next first stage writes the buffer the binning did not read
+      if (res == buf2) {
+        C *t = buf;
+        buf = buf2;
+        buf2 = t;
+      }
     }
   }
   __syncthreads();
@@ -524,17 +571,17 @@ static bool fused_reg1(const Plan &a, int B, bool pp) {
 
 // Persistent grid of the fused kernel: as many workgroups as stay resident
 // (occupancy of this instance x CUs), at most one per column block.
-template <typename T, int NF, bool REG1>
+template <typename T, int NF, bool REG1, bool R2L = false>
 static int fused_grid(const Plan &a, int B, int nrbins, bool pp) {
   int per_cu = 0;
   GENPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
-                                                          reinterpret_cast<const void *>(&k_xpass_bin<T, NF, REG1>),
+                                                          reinterpret_cast<const void *>(&k_xpass_bin<T, NF, REG1, R2L>),
                                                           BIN_THREADS, fused_lds<T, NF>(a, B, nrbins, pp)));
   const int64_t nblk = a.d * ceil_div(a.hz, B);
   return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)std::max(per_cu, 1) * num_cus(), nblk));
 }
 
-template <typename T, int NF, bool REG1>
+template <typename T, int NF, bool REG1, bool R2L = false>
 static void launch_fused(Plan &a, Plan *b, const BinTables &bt, int B, bool keep, bool pp, double *part, int G) {
   using C = typename Cx<T>::C;
   constexpr int NS = NF == 1 ? 1 : 3;
@@ -544,7 +591,7 @@ static void launch_fused(Plan &a, Plan *b, const BinTables &bt, int B, bool keep
   BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), bt.thr.as<int64_t>(), a.wtab.as<float>(), bt.lq};
   const size_t lds = fused_lds<T, NF>(a, B, bt.nrbins, pp);
   Stage st(a, "xpass_bin");
-  hipLaunchKernelGGL((k_xpass_bin<T, NF, REG1>), dim3(G), dim3(BIN_THREADS), lds, a.stream, a.field.as<C>(),
+  hipLaunchKernelGGL((k_xpass_bin<T, NF, REG1, R2L>), dim3(G), dim3(BIN_THREADS), lds, a.stream, a.field.as<C>(),
                      b ? b->field.as<C>() : (C *)nullptr, g, B, nkb, rp, tw, keep ? 1 : 0, pp ? 1 : 0, part);
   GENPK_LAUNCH_CHECK();
 }
@@ -579,15 +626,20 @@ void bin_spectra(Plan &a, Plan *b, int nrbins, bool fused_xpass, double *pow_aa,
     const bool pp = !b && (a.prec == 4 ? fused_lds<float, 1>(a, B, nrbins, true)
                                        : fused_lds<double, 1>(a, B, nrbins, true)) <= (size_t)LDS_LIMIT;
     const bool reg1 = !b && (a.prec == 4 ? fused_reg1<float>(a, B, pp) : fused_reg1<double>(a, B, pp));
+    // FP32 with a last stage of radix 2: that stage runs in the binning
+    const RadixPlan xrp = make_radix_plan((int)a.d);
+    const bool r2l = reg1 && a.prec == 4 && xrp.st[xrp.nst - 1].R == 2;
     if (a.prec == 4)
       G = b ? fused_grid<float, 2, false>(a, B, nrbins, false)
-            : reg1 ? fused_grid<float, 1, true>(a, B, nrbins, pp) : fused_grid<float, 1, false>(a, B, nrbins, pp);
+            : r2l ? fused_grid<float, 1, true, true>(a, B, nrbins, pp)
+                  : reg1 ? fused_grid<float, 1, true>(a, B, nrbins, pp) : fused_grid<float, 1, false>(a, B, nrbins, pp);
     else
       G = b ? fused_grid<double, 2, false>(a, B, nrbins, false)
             : reg1 ? fused_grid<double, 1, true>(a, B, nrbins, pp) : fused_grid<double, 1, false>(a, B, nrbins, pp);
     part = static_cast<double *>(a.bin_part.ensure((size_t)G * ns * nrbins * 8));
     if (a.prec == 4) {
       if (b) launch_fused<float, 2, false>(a, b, bt, B, false, false, part, G);
+      else if (r2l) launch_fused<float, 1, true, true>(a, b, bt, B, false, pp, part, G);
       else if (reg1) launch_fused<float, 1, true>(a, b, bt, B, false, pp, part, G);
       else launch_fused<float, 1, false>(a, b, bt, B, false, pp, part, G);
     } else {

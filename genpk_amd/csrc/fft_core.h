@@ -273,13 +273,14 @@ __device__ __forceinline__ void lds_fft(C *buf, const RadixPlan &rp, int nbatch,
     fft_stage_r<SIGN, BFAST, MAXE, false, C, PAD>(rp.st[s].R, buf, buf, nbatch, lgb, se, sb, rp.st[s], tw);
 }
 
-// Stages s0.. of a transform alternating between buffers a and b (one
-// barrier per stage); returns the buffer that holds the result.  The caller
-// has filled a and synchronised.
+// Stages s0 .. s1 - 1 (s1 < 0: to the last) of a transform alternating
+// between buffers a and b (one barrier per stage); returns the buffer that
+// holds the result.  The caller has filled a and synchronised.
 template <int SIGN, bool BFAST, int MAXE, typename C>
 __device__ __forceinline__ C *lds_fft_pp(C *a, C *b, const RadixPlan &rp, int nbatch, int lgb, int se, int sb,
-                                         const C *__restrict__ tw, int s0 = 0) {
-  for (int s = s0; s < rp.nst; s++) {
+                                         const C *__restrict__ tw, int s0 = 0, int s1 = -1) {
+  const int se1 = s1 < 0 ? rp.nst : s1;
+  for (int s = s0; s < se1; s++) {
     fft_stage_r<SIGN, BFAST, MAXE, true>(rp.st[s].R, a, b, nbatch, lgb, se, sb, rp.st[s], tw);
     C *t = a;
     a = b;
