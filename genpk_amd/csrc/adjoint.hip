@@ -116,13 +116,40 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_adjoint_x(typename Cx<T>::C *__
     for (int i = threadIdx.x; i < n; i += ADJ_THREADS) stw[i] = tw[i];
     fft_first_regs<MAXE, -1>(v, buf2, B, lgb, B, 1);
     __syncthreads();
-    C *res = lds_fft_pp<-1, true, MAXE>(buf2, buf, rp, B, lgb, B, 1, stw, 1);
-    for (int i = threadIdx.x; i < tot; i += ADJ_THREADS) {
-      const int e = i >> lgb, b = i & (B - 1);
-      if (b < nb) {
-        const double s = w.at(e, y, kz0 + b, d);
-        const C a = res[i];
-        res[i] = C{(T)((double)a.x * s), (T)((double)a.y * s)};
+    C *res;
+    if (rp.st[rp.nst - 1].R == 2) {
+      // the last forward stage (radix 2) and the adjoint weight in registers:
+      // butterfly jj pairs stage inputs jj and jj + n/2 (the latter times
+      // w^jj) into modes jj and jj + n/2, written weighted to the other buffer
+      C *x = lds_fft_pp<-1, true, MAXE>(buf2, buf, rp, B, lgb, B, 1, stw, 1, rp.nst - 1);
+      res = x == buf ? buf2 : buf;
+      constexpr int HB = MAXE / 2;
+      const int half = n / 2;
+      const int b = threadIdx.x & (B - 1);
+      const int jb = (threadIdx.x >> lgb) * HB;
+#pragma unroll
+      for (int q = 0; q < HB; q++) {
+        const int jj = jb + q;
+        const C a0 = x[jj * B + b];
+        const C v1 = cmul(x[(jj + half) * B + b], stw[jj]);
+        C o0 = cadd(a0, v1), o1 = csub(a0, v1);
+        if (b < nb) {
+          const double s0 = w.at(jj, y, kz0 + b, d), s1 = w.at(jj + half, y, kz0 + b, d);
+          o0 = C{(T)((double)o0.x * s0), (T)((double)o0.y * s0)};
+          o1 = C{(T)((double)o1.x * s1), (T)((double)o1.y * s1)};
+        }
+        res[jj * B + b] = o0;
+        res[(jj + half) * B + b] = o1;
+      }
+    } else {
+      res = lds_fft_pp<-1, true, MAXE>(buf2, buf, rp, B, lgb, B, 1, stw, 1);
+      for (int i = threadIdx.x; i < tot; i += ADJ_THREADS) {
+        const int e = i >> lgb, b = i & (B - 1);
+        if (b < nb) {
+          const double s = w.at(e, y, kz0 + b, d);
+          const C a = res[i];
+          res[i] = C{(T)((double)a.x * s), (T)((double)a.y * s)};
+        }
       }
     }
     __syncthreads();
