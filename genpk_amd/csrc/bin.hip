@@ -585,6 +585,8 @@ template <typename T, int NF, bool REG1, bool R2L = false>
 static void launch_fused(Plan &a, Plan *b, const BinTables &bt, int B, bool keep, bool pp, double *part, int G) {
   using C = typename Cx<T>::C;
   constexpr int NS = NF == 1 ? 1 : 3;
+  // the R2L instance finishes the transform only in registers for the binning
+  GENPK_REQUIRE(!(R2L && keep), "the fused pass cannot keep the spectrum with the last stage in the binning");
   const int nkb = (int)ceil_div(a.hz, B);
   const RadixPlan rp = make_radix_plan((int)a.d);
   const C *tw = (a.prec == 4) ? (const C *)a.fft.tw_d_f32.ptr : (const C *)a.fft.tw_d_f64.ptr;
