@@ -224,8 +224,17 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const uint32_t *__restr
     if (c >= cells) continue;
     const uint32_t m = item_start[k + 1] - item_start[k];
     const double *pp = partials + (int64_t)part_start[k] * cells + c;
+    // eight loads in flight per thread; the additions keep the order j = 0, 1, ...
     double s = 0;
-    for (uint32_t j = 0; j < m; j++) s += pp[(int64_t)j * cells];
+    uint32_t j = 0;
+    for (; j + 8 <= m; j += 8) {
+      double t[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) t[q] = pp[(int64_t)(j + q) * cells];
+#pragma unroll
+      for (int q = 0; q < 8; q++) s += t[q];
+    }
+    for (; j < m; j++) s += pp[(int64_t)j * cells];
     const int64_t row = c / P.HZ;
     const int lz = (int)(c - row * P.HZ);
     Real *out = lz < P.kg.TZ ? tiles + (int64_t)k * P.cm + row * P.kg.TZ + lz : tiles + P.zoff + (int64_t)k * P.cz + row;

@@ -359,6 +359,62 @@ __global__ void __launch_bounds__(COL_THREADS) k_cols(typename Cx<T>::C *__restr
   }
 }
 
+// k_cols for n * B = COL_THREADS * COLE and a first radix of COLE: every
+// thread loads exactly the inputs of one first-stage butterfly (elements
+// j + q * n/R of column b, j = tid >> lgb, b = tid & (B-1): the same
+// addresses as k_cols' element-wise load), so the first stage runs from
+// registers and the load's LDS round trip and two barriers are saved.
+// L2 (last radix 2): the last stage's butterflies (rows k and k + n/2 of a
+// column, twiddle w^k) write their outputs straight to global memory.
+template <typename T, int SIGN, bool L2>
+__global__ void __launch_bounds__(COL_THREADS) k_cols_r1(typename Cx<T>::C *__restrict__ f, int64_t d, int64_t hz,
+                                                        int lgb, int nkb, int axis, RadixPlan rp,
+                                                        const typename Cx<T>::C *__restrict__ tw) {
+  using C = typename Cx<T>::C;
+  constexpr int R = FftCfg<T>::COLE;
+  extern __shared__ unsigned char lds_raw[];
+  C *buf = reinterpret_cast<C *>(lds_raw);  // [n][B]
+  const int n = rp.n;
+  const int B = 1 << lgb;
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t outer = blk / nkb;
+  const int64_t kz0 = (int64_t)(blk - outer * nkb) * B;
+  const int nb = (int)min((int64_t)B, hz - kz0);
+  const int64_t estride = axis == 1 ? hz : d * hz;
+  C *base = f + (axis == 1 ? outer * d * hz : outer * hz) + kz0;
+  const int b = threadIdx.x & (B - 1), j = threadIdx.x >> lgb, nq = n / R;
+  C v[R];
+#pragma unroll
+  for (int q = 0; q < R; q++) v[q] = b < nb ? base[(int64_t)(j + q * nq) * estride + b] : C{0, 0};
+  C *stw = buf + (int64_t)n * B;  // twiddles staged in LDS
+  for (int i = threadIdx.x; i < n; i += COL_THREADS) stw[i] = tw[i];
+  fft_first_regs<R, SIGN>(v, buf, B, lgb, B, 1);
+  __syncthreads();
+  for (int s = 1; s < rp.nst - (L2 ? 1 : 0); s++)
+    fft_stage_r<SIGN, true, R, false, C, false>(rp.st[s].R, buf, buf, B, lgb, B, 1, rp.st[s], stw);
+  if constexpr (L2) {
+    const int h = n / 2;
+#pragma unroll
+    for (int i = 0; i < R / 2; i++) {
+      const int k = j + i * (COL_THREADS >> lgb);  // butterfly b + k * B of B * n/2
+      C t = stw[k];
+      if (SIGN > 0) t.y = -t.y;
+      C w[2] = {buf[k * B + b], cmul(buf[(k + h) * B + b], t)};
+      Dft<2, SIGN, C>::run(w);
+      if (b < nb) {
+        base[(int64_t)k * estride + b] = w[0];
+        base[(int64_t)(k + h) * estride + b] = w[1];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      const int e = (threadIdx.x + q * COL_THREADS) >> lgb;
+      if (b < nb) base[(int64_t)e * estride + b] = buf[threadIdx.x + q * COL_THREADS];
+    }
+  }
+}
+
 template <typename T>
 int rows_per_block(int64_t M) {
   return (int)std::max<int64_t>(1, FftCfg<T>::ELEMS / std::max<int64_t>(M + 1, 1));
@@ -462,9 +518,20 @@ static void cols_impl(Plan &p, int axis) {
   const int nkb = (int)ceil_div(p.hz, B);
   const RadixPlan rp = make_radix_plan((int)p.d);
   const C *tw = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
+  const size_t lds = (size_t)p.d * (B + 1) * sizeof(C);
+  const int64_t nblk = p.d * nkb;
   Stage st(p, axis == 1 ? (SIGN < 0 ? "fft_y" : "ifft_y") : (SIGN < 0 ? "fft_x" : "ifft_x"));
-  hipLaunchKernelGGL((k_cols<T, SIGN>), dim3((unsigned)(p.d * nkb)), dim3(COL_THREADS), (size_t)p.d * (B + 1) * sizeof(C),
-                     p.stream, p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, axis, rp, tw);
+  if (rp.st[0].R == FftCfg<T>::COLE && p.d * B == (int64_t)COL_THREADS * FftCfg<T>::COLE) {
+    if (rp.st[rp.nst - 1].R == 2)
+      hipLaunchKernelGGL((k_cols_r1<T, SIGN, true>), dim3((unsigned)nblk), dim3(COL_THREADS), lds, p.stream,
+                         p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, axis, rp, tw);
+    else
+      hipLaunchKernelGGL((k_cols_r1<T, SIGN, false>), dim3((unsigned)nblk), dim3(COL_THREADS), lds, p.stream,
+                         p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, axis, rp, tw);
+  } else {
+    hipLaunchKernelGGL((k_cols<T, SIGN>), dim3((unsigned)nblk), dim3(COL_THREADS), lds, p.stream, p.field.as<C>(), p.d,
+                       p.hz, ilog2(B), nkb, axis, rp, tw);
+  }
   GENPK_LAUNCH_CHECK();
 }
 
