@@ -75,7 +75,7 @@ def test_fft_shim_vs_oracle(gpu, d):
 
 
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-13), ("f32", 3e-6)])
-@pytest.mark.parametrize("d", [16, 64, 128, 384])
+@pytest.mark.parametrize("d", [16, 64, 128, 256, 384])
 def test_r2c_c2r_plan(gpu, d, prec, tol):
     import torch
     announce("r2c_c2r", d, prec)
