@@ -19,6 +19,7 @@
 // (the deposit's partition), reads dL/drho from a halo'd tile staged in LDS
 // and writes the gradients in tile order; the sort's permutations then take
 // them back to the particles' order.
+#include "binlookup.h"
 #include "fft_core.h"
 #include "tilesort.h"
 
@@ -45,6 +46,9 @@ struct AdjWeight {
   const double *lambda;
   const float *wtab;
   double inv_m2;
+  const int64_t *thr;  // bin thresholds [nrbins + 1] (smallest q whose bin is >= b)
+  int nrbins;
+  float lq;            // bin(q) ~ lq * log2(q)
   // 2 lambda_b g_k / (M^2 C_b) of the mode at array indices (x, y, kz); 0 at k = 0
   __device__ __forceinline__ double at(int64_t x, int64_t y, int64_t kz, int64_t d) const {
     const int64_t kx = kv(x, d), ky = kv(y, d);
@@ -78,7 +82,11 @@ __global__ void k_weight(typename Cx<T>::C *__restrict__ f, int64_t d, int64_t h
 // ADJ_THREADS * MAXE and a first stage of radix MAXE): the first stage runs
 // from the loaded registers and the stages alternate between two LDS
 // buffers (one barrier each).
-template <typename T, bool PP>
+// LT (with PP and a last radix 2): the per-bin factors 2 lambda_b / (M^2 C_b),
+// the window factors and the bin thresholds are staged in LDS; each thread's
+// modes find their bin by tracking the thresholds along kx, so a mode's
+// weight costs no global read and no division.
+template <typename T, bool PP, bool LT = false>
 __global__ void __launch_bounds__(ADJ_THREADS) k_adjoint_x(typename Cx<T>::C *__restrict__ f, int64_t d, int64_t hz,
                                                           int lgb, int nkb, RadixPlan rp,
                                                           const typename Cx<T>::C *__restrict__ tw, AdjWeight w) {
@@ -114,6 +122,17 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_adjoint_x(typename Cx<T>::C *__
     C *buf2 = buf + tot;
     C *stw = buf2 + tot;
     for (int i = threadIdx.x; i < n; i += ADJ_THREADS) stw[i] = tw[i];
+    double *sfb = reinterpret_cast<double *>(stw + n);  // [nrbins]
+    float *swt = reinterpret_cast<float *>(sfb + w.nrbins);  // [d]
+    int *sthr = reinterpret_cast<int *>(swt + d);  // [nrbins + 1]
+    if constexpr (LT) {
+      for (int i = threadIdx.x; i < w.nrbins; i += ADJ_THREADS) {
+        const int c = w.count[i];
+        sfb[i] = c ? 2.0 * w.lambda[i] * w.inv_m2 / (double)c : 0.0;
+      }
+      for (int i = threadIdx.x; i < d; i += ADJ_THREADS) swt[i] = w.wtab[i];
+      for (int i = threadIdx.x; i <= w.nrbins; i += ADJ_THREADS) sthr[i] = (int)w.thr[i];
+    }
     fft_first_regs<MAXE, -1>(v, buf2, B, lgb, B, 1);
     __syncthreads();
     C *res;
@@ -127,6 +146,21 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_adjoint_x(typename Cx<T>::C *__
       const int half = n / 2;
       const int b = threadIdx.x & (B - 1);
       const int jb = (threadIdx.x >> lgb) * HB;
+      // LT: modes jj (kx = jj) and jj + n/2 (kx = jj - n/2, kx = n/2 at jj = 0)
+      // each keep a bin tracker along their run
+      const int kz = (int)kz0 + b, ky = (int)kv(y, d);
+      const int a = ky * ky + kz * kz;
+      const float wy = LT ? swt[y] : 0.f, wz = LT ? swt[min(kz, (int)d - 1)] : 0.f;
+      BinTrackLds t0, t1;
+      auto lds_weight = [&](int xi, BinTrackLds &bt) -> double {
+        const int kx = xi <= half ? xi : xi - n;
+        const int qk = a + kx * kx;
+        if (qk == 0) return 0.0;
+        const float gw = (swt[xi] * wy) * wz;  // the forward's ((wx*wy)*wz) in float
+        double g = (double)gw * (double)gw;
+        g *= g;
+        return g * sfb[bt.of(qk, sthr, w.nrbins, w.lq)];
+      };
 #pragma unroll
       for (int q = 0; q < HB; q++) {
         const int jj = jb + q;
@@ -134,7 +168,8 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_adjoint_x(typename Cx<T>::C *__
         const C v1 = cmul(x[(jj + half) * B + b], stw[jj]);
         C o0 = cadd(a0, v1), o1 = csub(a0, v1);
         if (b < nb) {
-          const double s0 = w.at(jj, y, kz0 + b, d), s1 = w.at(jj + half, y, kz0 + b, d);
+          const double s0 = LT ? lds_weight(jj, t0) : w.at(jj, y, kz0 + b, d);
+          const double s1 = LT ? lds_weight(jj + half, t1) : w.at(jj + half, y, kz0 + b, d);
           o0 = C{(T)((double)o0.x * s0), (T)((double)o0.y * s0)};
           o1 = C{(T)((double)o1.x * s1), (T)((double)o1.y * s1)};
         }
@@ -295,6 +330,9 @@ AdjWeight adj_weight(Plan &p, int nrbins, const double *lambda, double m) {
   w.lambda = lambda;
   w.wtab = p.wtab.as<float>();
   w.inv_m2 = 1.0 / (m * m);
+  w.thr = bt.thr.as<int64_t>();
+  w.nrbins = bt.nrbins;
+  w.lq = bt.lq;
   return w;
 }
 
@@ -311,7 +349,14 @@ void adjoint_x_impl(Plan &p, const AdjWeight &w) {
   const bool pp = rp.st[0].R == AdjCfg<T>::MAXE && tot == (int64_t)ADJ_THREADS * AdjCfg<T>::MAXE &&
                   (size_t)(2 * tot + p.d) * sizeof(C) <= 160 * 1024;
   const dim3 grid((unsigned)(p.d * nkb));
-  if (pp)
+  // the LDS tables of the fused weight: per-bin factors, window factors, thresholds
+  const size_t lds_lt = (size_t)(2 * tot + p.d) * sizeof(C) + (size_t)w.nrbins * 8 + (size_t)p.d * 4 +
+                        (size_t)(w.nrbins + 1) * 4;
+  const bool lt = pp && rp.st[rp.nst - 1].R == 2 && w.nrbins >= 1 && lds_lt <= 160 * 1024;
+  if (lt)
+    hipLaunchKernelGGL((k_adjoint_x<T, true, true>), grid, dim3(ADJ_THREADS), lds_lt, p.stream, p.field.as<C>(), p.d,
+                       p.hz, ilog2(B), nkb, rp, tw, w);
+  else if (pp)
     hipLaunchKernelGGL((k_adjoint_x<T, true>), grid, dim3(ADJ_THREADS), (size_t)(2 * tot + p.d) * sizeof(C), p.stream,
                        p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, rp, tw, w);
   else
