@@ -379,9 +379,25 @@ __global__ void k_bin_finish(int nrbins, int ns, int G, const double *__restrict
                              double *__restrict__ p_ab, int *__restrict__ count_out, double *__restrict__ keffs_out) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nrbins) return;
+  // eight partials in flight per thread; each sum adds in the order gi = 0, 1, ...
   double s[3] = {0, 0, 0};
-  for (int gi = 0; gi < G; gi++)
-    for (int k = 0; k < ns; k++) s[k] += part[((int64_t)gi * ns + k) * nrbins + b];
+  const int64_t gs = (int64_t)ns * nrbins;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    if (k >= ns) break;
+    const double *pk = part + (int64_t)k * nrbins + b;
+    double acc = 0;
+    int gi = 0;
+    for (; gi + 8 <= G; gi += 8) {
+      double t[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) t[q] = pk[(int64_t)(gi + q) * gs];
+#pragma unroll
+      for (int q = 0; q < 8; q++) acc += t[q];
+    }
+    for (; gi < G; gi++) acc += pk[(int64_t)gi * gs];
+    s[k] = acc;
+  }
   const int c = count[b];
   double *outs[3] = {p_aa, p_bb, p_ab};
   const double ms[3] = {m_aa, m_bb, m_ab};
