@@ -254,52 +254,78 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather_tiles(const T *__rest
   const int ty = (int)((k / ntz) % nty);
   const int tx = (int)(k / ((int64_t)ntz * nty));
   const int ox = tx << P.kg.sx, oy = ty << P.kg.sy, oz = tz << P.kg.sz;
-  for (int c = threadIdx.x; c < (int)P.cells; c += GATHER_THREADS) {
-    const int row = (int)P.fhz.div((uint32_t)c);
-    const int lz = c - row * P.HZ;
-    const int lx = (int)P.fhy.div((uint32_t)row);
-    const int ly = row - lx * P.HY;
-    int x = ox + lx, y = oy + ly, z = oz + lz;
-    if (x >= d) x -= d;
-    if (y >= d) y -= d;
-    if (z >= d) z -= d;
-    g[c] = G[((int64_t)x * d + y) * P.rowlen + z];
+  // four loads in flight per thread while staging, and four records while gathering
+  constexpr int UNR = 4;
+  for (int c0 = threadIdx.x; c0 < (int)P.cells; c0 += GATHER_THREADS * UNR) {
+    T t[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const int c = c0 + u * GATHER_THREADS;
+      if (c < (int)P.cells) {
+        const int row = (int)P.fhz.div((uint32_t)c);
+        const int lz = c - row * P.HZ;
+        const int lx = (int)P.fhy.div((uint32_t)row);
+        const int ly = row - lx * P.HY;
+        int x = ox + lx, y = oy + ly, z = oz + lz;
+        if (x >= d) x -= d;
+        if (y >= d) y -= d;
+        if (z >= d) z -= d;
+        t[u] = G[((int64_t)x * d + y) * P.rowlen + z];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const int c = c0 + u * GATHER_THREADS;
+      if (c < (int)P.cells) g[c] = t[u];
+    }
   }
   __syncthreads();
   const int sy = P.HZ, sx = P.HY * P.HZ;
   const double units = P.kg.units;
-  for (uint32_t i = ir.pb + threadIdx.x; i < ir.pe; i += GATHER_THREADS) {
-    float r[RW];
+  for (uint32_t i0 = ir.pb + threadIdx.x; i0 < ir.pe; i0 += GATHER_THREADS * UNR) {
+    float rr[UNR][RW];
 #pragma unroll
-    for (int q = 0; q < RW; q++) r[q] = rec[(int64_t)i * RW + q];
-    double fx, fy, fz;
-    const int lx = cell_of(r[0], units, d, fx) - ox;
-    const int ly = cell_of(r[1], units, d, fy) - oy;
-    const int lz = cell_of(r[2], units, d, fz) - oz;
-    if ((unsigned)lx >= (unsigned)P.kg.TX || (unsigned)ly >= (unsigned)P.kg.TY || (unsigned)lz >= (unsigned)P.kg.TZ)
-      continue;  // cannot happen for records of this tile; guards the LDS bounds
-    const int at = lx * sx + ly * sy + lz;
-    const double g000 = (double)g[at], g001 = (double)g[at + 1];
-    const double g100 = (double)g[at + sx], g101 = (double)g[at + sx + 1];
-    const double g010 = (double)g[at + sy], g011 = (double)g[at + sy + 1];
-    const double g110 = (double)g[at + sx + sy], g111 = (double)g[at + sx + sy + 1];
-    // corners (bx, by, bz); weights (1-f or f) per axis
-    const double ex = 1.0 - fx, ey = 1.0 - fy, ez = 1.0 - fz;
-    // interpolate along z first
-    const double a00 = ez * g000 + fz * g001, a10 = ez * g100 + fz * g101;
-    const double a01 = ez * g010 + fz * g011, a11 = ez * g110 + fz * g111;
-    const double dgx = ey * (a10 - a00) + fy * (a11 - a01);
-    const double dgy = ex * (a01 - a00) + fx * (a11 - a10);
-    const double b00 = ex * ey, b10 = fx * ey, b01 = ex * fy, b11 = fx * fy;
-    const double dgz = b00 * (g001 - g000) + b10 * (g101 - g100) + b01 * (g011 - g010) + b11 * (g111 - g110);
-    const double gm = b00 * a00 + b10 * a10 + b01 * a01 + b11 * a11;
-    const double m = NM ? (double)r[3] : P.mass;
-    const double s = m * units;
-    float *o = out + (int64_t)i * RWO;
-    o[0] = (float)(dgx * s);
-    o[1] = (float)(dgy * s);
-    o[2] = (float)(dgz * s);
-    if (GM) o[3] = (float)gm;
+    for (int u = 0; u < UNR; u++) {
+      const uint32_t i = i0 + u * GATHER_THREADS;
+      if (i < ir.pe) {
+#pragma unroll
+        for (int q = 0; q < RW; q++) rr[u][q] = rec[(int64_t)i * RW + q];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const uint32_t i = i0 + u * GATHER_THREADS;
+      if (i >= ir.pe) break;
+      const float *r = rr[u];
+      double fx, fy, fz;
+      const int lx = cell_of(r[0], units, d, fx) - ox;
+      const int ly = cell_of(r[1], units, d, fy) - oy;
+      const int lz = cell_of(r[2], units, d, fz) - oz;
+      if ((unsigned)lx >= (unsigned)P.kg.TX || (unsigned)ly >= (unsigned)P.kg.TY || (unsigned)lz >= (unsigned)P.kg.TZ)
+        continue;  // cannot happen for records of this tile; guards the LDS bounds
+      const int at = lx * sx + ly * sy + lz;
+      const double g000 = (double)g[at], g001 = (double)g[at + 1];
+      const double g100 = (double)g[at + sx], g101 = (double)g[at + sx + 1];
+      const double g010 = (double)g[at + sy], g011 = (double)g[at + sy + 1];
+      const double g110 = (double)g[at + sx + sy], g111 = (double)g[at + sx + sy + 1];
+      // corners (bx, by, bz); weights (1-f or f) per axis
+      const double ex = 1.0 - fx, ey = 1.0 - fy, ez = 1.0 - fz;
+      // interpolate along z first
+      const double a00 = ez * g000 + fz * g001, a10 = ez * g100 + fz * g101;
+      const double a01 = ez * g010 + fz * g011, a11 = ez * g110 + fz * g111;
+      const double dgx = ey * (a10 - a00) + fy * (a11 - a01);
+      const double dgy = ex * (a01 - a00) + fx * (a11 - a10);
+      const double b00 = ex * ey, b10 = fx * ey, b01 = ex * fy, b11 = fx * fy;
+      const double dgz = b00 * (g001 - g000) + b10 * (g101 - g100) + b01 * (g011 - g010) + b11 * (g111 - g110);
+      const double gm = b00 * a00 + b10 * a10 + b01 * a01 + b11 * a11;
+      const double m = NM ? (double)r[3] : P.mass;
+      const double s = m * units;
+      float *o = out + (int64_t)i * RWO;
+      o[0] = (float)(dgx * s);
+      o[1] = (float)(dgy * s);
+      o[2] = (float)(dgz * s);
+      if (GM) o[3] = (float)gm;
+    }
   }
 }
 
@@ -309,16 +335,16 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather_tiles(const T *__rest
 // cost far less than scattered partial-line writes.
 template <int RWO, int TWO>
 __global__ void k_unpermute(const float *__restrict__ rec, const uint32_t *__restrict__ perm1,
-                            const uint32_t *__restrict__ perm2, int64_t n, float *__restrict__ gpos,
-                            float *__restrict__ gmass) {
+                              const uint32_t *__restrict__ perm2, int64_t n, float *__restrict__ gpos,
+                              float *__restrict__ gmass) {
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
-    int64_t at = perm1[j];
-    if (TWO) at = perm2[at];
-    const float *r = rec + at * RWO;
-    gpos[3 * j + 0] = r[0];
-    gpos[3 * j + 1] = r[1];
-    gpos[3 * j + 2] = r[2];
-    if (RWO == 4) gmass[j] = r[3];
+      int64_t at = perm1[j];
+      if (TWO) at = perm2[at];
+      const float *r = rec + at * RWO;
+      gpos[3 * j + 0] = r[0];
+      gpos[3 * j + 1] = r[1];
+      gpos[3 * j + 2] = r[2];
+      if (RWO == 4) gmass[j] = r[3];
   }
 }
 
@@ -347,21 +373,21 @@ void adjoint_x_impl(Plan &p, const AdjWeight &w) {
   Stage st(p, "adjoint_x");
   const int64_t tot = p.d * B;
   const bool pp = rp.st[0].R == AdjCfg<T>::MAXE && tot == (int64_t)ADJ_THREADS * AdjCfg<T>::MAXE &&
-                  (size_t)(2 * tot + p.d) * sizeof(C) <= 160 * 1024;
+                    (size_t)(2 * tot + p.d) * sizeof(C) <= 160 * 1024;
   const dim3 grid((unsigned)(p.d * nkb));
   // the LDS tables of the fused weight: per-bin factors, window factors, thresholds
   const size_t lds_lt = (size_t)(2 * tot + p.d) * sizeof(C) + (size_t)w.nrbins * 8 + (size_t)p.d * 4 +
-                        (size_t)(w.nrbins + 1) * 4;
+                          (size_t)(w.nrbins + 1) * 4;
   const bool lt = pp && rp.st[rp.nst - 1].R == 2 && w.nrbins >= 1 && lds_lt <= 160 * 1024;
   if (lt)
-    hipLaunchKernelGGL((k_adjoint_x<T, true, true>), grid, dim3(ADJ_THREADS), lds_lt, p.stream, p.field.as<C>(), p.d,
-                       p.hz, ilog2(B), nkb, rp, tw, w);
+      hipLaunchKernelGGL((k_adjoint_x<T, true, true>), grid, dim3(ADJ_THREADS), lds_lt, p.stream, p.field.as<C>(), p.d,
+                         p.hz, ilog2(B), nkb, rp, tw, w);
   else if (pp)
-    hipLaunchKernelGGL((k_adjoint_x<T, true>), grid, dim3(ADJ_THREADS), (size_t)(2 * tot + p.d) * sizeof(C), p.stream,
-                       p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, rp, tw, w);
+      hipLaunchKernelGGL((k_adjoint_x<T, true>), grid, dim3(ADJ_THREADS), (size_t)(2 * tot + p.d) * sizeof(C), p.stream,
+                         p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, rp, tw, w);
   else
-    hipLaunchKernelGGL((k_adjoint_x<T, false>), grid, dim3(ADJ_THREADS), (size_t)(tot + p.d) * sizeof(C), p.stream,
-                       p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, rp, tw, w);
+      hipLaunchKernelGGL((k_adjoint_x<T, false>), grid, dim3(ADJ_THREADS), (size_t)(tot + p.d) * sizeof(C), p.stream,
+                         p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, rp, tw, w);
   GENPK_LAUNCH_CHECK();
 }
 
