@@ -335,16 +335,16 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather_tiles(const T *__rest
 // cost far less than scattered partial-line writes.
 template <int RWO, int TWO>
 __global__ void k_unpermute(const float *__restrict__ rec, const uint32_t *__restrict__ perm1,
-                              const uint32_t *__restrict__ perm2, int64_t n, float *__restrict__ gpos,
-                              float *__restrict__ gmass) {
+                            const uint32_t *__restrict__ perm2, int64_t n, float *__restrict__ gpos,
+                            float *__restrict__ gmass) {
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
-      int64_t at = perm1[j];
-      if (TWO) at = perm2[at];
-      const float *r = rec + at * RWO;
-      gpos[3 * j + 0] = r[0];
-      gpos[3 * j + 1] = r[1];
-      gpos[3 * j + 2] = r[2];
-      if (RWO == 4) gmass[j] = r[3];
+    int64_t at = perm1[j];
+    if (TWO) at = perm2[at];
+    const float *r = rec + at * RWO;
+    gpos[3 * j + 0] = r[0];
+    gpos[3 * j + 1] = r[1];
+    gpos[3 * j + 2] = r[2];
+    if (RWO == 4) gmass[j] = r[3];
   }
 }
 
@@ -373,21 +373,21 @@ void adjoint_x_impl(Plan &p, const AdjWeight &w) {
   Stage st(p, "adjoint_x");
   const int64_t tot = p.d * B;
   const bool pp = rp.st[0].R == AdjCfg<T>::MAXE && tot == (int64_t)ADJ_THREADS * AdjCfg<T>::MAXE &&
-                    (size_t)(2 * tot + p.d) * sizeof(C) <= 160 * 1024;
+                  (size_t)(2 * tot + p.d) * sizeof(C) <= 160 * 1024;
   const dim3 grid((unsigned)(p.d * nkb));
   // the LDS tables of the fused weight: per-bin factors, window factors, thresholds
   const size_t lds_lt = (size_t)(2 * tot + p.d) * sizeof(C) + (size_t)w.nrbins * 8 + (size_t)p.d * 4 +
-                          (size_t)(w.nrbins + 1) * 4;
+                        (size_t)(w.nrbins + 1) * 4;
   const bool lt = pp && rp.st[rp.nst - 1].R == 2 && w.nrbins >= 1 && lds_lt <= 160 * 1024;
   if (lt)
-      hipLaunchKernelGGL((k_adjoint_x<T, true, true>), grid, dim3(ADJ_THREADS), lds_lt, p.stream, p.field.as<C>(), p.d,
-                         p.hz, ilog2(B), nkb, rp, tw, w);
+    hipLaunchKernelGGL((k_adjoint_x<T, true, true>), grid, dim3(ADJ_THREADS), lds_lt, p.stream, p.field.as<C>(), p.d,
+                       p.hz, ilog2(B), nkb, rp, tw, w);
   else if (pp)
-      hipLaunchKernelGGL((k_adjoint_x<T, true>), grid, dim3(ADJ_THREADS), (size_t)(2 * tot + p.d) * sizeof(C), p.stream,
-                         p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, rp, tw, w);
+    hipLaunchKernelGGL((k_adjoint_x<T, true>), grid, dim3(ADJ_THREADS), (size_t)(2 * tot + p.d) * sizeof(C), p.stream,
+                       p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, rp, tw, w);
   else
-      hipLaunchKernelGGL((k_adjoint_x<T, false>), grid, dim3(ADJ_THREADS), (size_t)(tot + p.d) * sizeof(C), p.stream,
-                         p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, rp, tw, w);
+    hipLaunchKernelGGL((k_adjoint_x<T, false>), grid, dim3(ADJ_THREADS), (size_t)(tot + p.d) * sizeof(C), p.stream,
+                       p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, rp, tw, w);
   GENPK_LAUNCH_CHECK();
 }
 
