@@ -422,6 +422,32 @@ def test_counts_1024_reference_facts(gpu):
     assert int(c.sum()) == 1024 ** 3 - 1
 
 
+def test_r2c_from_tiles_1024_matches_density_path(gpu):
+    """d = 1024, FP32: the z pass that reads the tile store (the eight inputs
+    of a first-stage butterfly per thread straight from the store, z halos
+    added on load) gives the half spectrum of the element-wise z pass on the
+    exported density, the same arithmetic in another order of work: within
+    1e-6 of the largest |F| (a wrong index or halo would differ at O(1))."""
+    import torch
+    announce("r2c from tiles 1024")
+    d, box, n = 1024, 1000.0, 1 << 20
+    pos = synth.clumped_torch(n, box, d, n_clumps=64, seed=41, device="cuda")
+    p = genpk_amd.Plan(d, "f32")
+    p.deposit(pos, box)
+    dens = p.export_density()
+    F1 = p.r2c().clone()
+    del p
+    q = genpk_amd.Plan(d, "f32")
+    q.import_density(dens)
+    F2 = q.r2c()
+    scale = float(F2.abs().max())
+    diff = float((F1 - F2).abs().max())
+    print(f"max |dF| {diff:.3e} of {scale:.3e}", flush=True)
+    assert diff <= 1e-6 * scale
+    del q, F1, F2, dens
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("with_masses", [False, True])
 def test_power_1024_fp32_vs_fp64(gpu, with_masses):
     """d = 1024, where the FP32 forward reads a z row one element per thread
