@@ -6,7 +6,8 @@
 //    accumulates the 8 CIC weights of each particle into the halo'd tile in
 //    LDS as 64-bit fixed point (ds_add_u64), then flushes the whole tile with
 //    coalesced stores.  Tiles with more than DEPOSIT_ITEM_MAX particles are
-//    split; their items write FP64 partial tiles that a reduction sums.
+//    split; their items write their raw accumulators as partial tiles, which a
+//    reduction sums exactly (fixed point) whatever records formed each item.
 // 3. Fold.  Grid cell values are assembled from the tile store by adding the
 //    halo cells of the previous tiles onto each tile's low faces: y and x
 //    halos in place after the deposit (k_fold_axis), z halos by the readers
@@ -198,8 +199,13 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
           zh[c] = (Real)((double)zh[c] + cell(c * P.HZ + TZ));
       }
     } else {
-      double *out = partials + (int64_t)(part_start[k] + j) * P.cells;
-      for (int64_t c = threadIdx.x; c < P.cells; c += DEP_THREADS) out[c] = cell(c);
+      // a split tile: this item's raw accumulators (fixed-point integers, or
+      // the FP64 sums), which k_reduce_partials adds up
+      Acc *out = reinterpret_cast<Acc *>(partials) + (int64_t)(part_start[k] + j) * P.cells;
+      for (int64_t c = threadIdx.x; c < P.cells; c += DEP_THREADS) {
+        out[c] = acc[c];
+        acc[c] = 0;
+      }
     }
     __syncthreads();  // the flush has read and cleared the tile before the next item adds to it
     ir = nir;
@@ -207,14 +213,26 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
   }
 }
 
-// Sum the partial tiles of split tiles into the tile store.
-template <typename Real>
+// Sum the partial tiles of split tiles into the tile store.  Fixed-point
+// partials add exactly in a 128-bit integer and convert once, so a cell's
+// value does not depend on which records of the tile formed which item (the
+// order of the records within a tile is not fixed); FP64 partials (masses
+// beyond the fixed-point range) add in item order.
+template <typename Real, int NM>
 __global__ void __launch_bounds__(256) k_reduce_partials(const uint32_t *__restrict__ nheavy,
                                                          const uint32_t *__restrict__ heavy,
                                                          const uint32_t *__restrict__ item_start,
                                                          const uint32_t *__restrict__ part_start,
                                                          const double *__restrict__ partials, DepParams P,
                                                          Real *__restrict__ tiles) {
+  bool fixed = true;
+  double scale = P.mass * FIXED_INV;  // constant mass: unsigned sums of 2^36 per unit weight
+  if constexpr (NM == 1) {
+    const uint32_t mxb = P.mrange[0], mnb = P.mrange[1];
+    fixed = mxb < 0x7f800000u && (mxb == 0u || (int)(mxb >> 23) - (int)(mnb >> 23) <= 10);
+    scale = ldexp(1.0, -(45 - ((int)(mxb >> 23) - 126)));  // signed sums of 2^s per unit mass
+  }
+  const uint64_t *raw = reinterpret_cast<const uint64_t *>(partials);
   const int64_t cells = P.cells;
   const int64_t nblk = ceil_div(cells, 256);
   const int64_t units = (int64_t)(*nheavy) * nblk;
@@ -223,18 +241,45 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const uint32_t *__restr
     const int64_t c = (u % nblk) * 256 + threadIdx.x;
     if (c >= cells) continue;
     const uint32_t m = item_start[k + 1] - item_start[k];
-    const double *pp = partials + (int64_t)part_start[k] * cells + c;
-    // eight loads in flight per thread; the additions keep the order j = 0, 1, ...
-    double s = 0;
-    uint32_t j = 0;
-    for (; j + 8 <= m; j += 8) {
-      double t[8];
+    const int64_t at = (int64_t)part_start[k] * cells + c;
+    double s;
+    if (fixed) {
+      // 128-bit two's-complement sum (hi:lo); eight loads in flight per thread
+      uint64_t lo = 0, hi = 0;
+      auto add = [&](uint64_t v) {
+        lo += v;
+        hi += (lo < v ? 1u : 0u) + (NM == 1 && (int64_t)v < 0 ? ~0ull : 0ull);
+      };
+      uint32_t j = 0;
+      for (; j + 8 <= m; j += 8) {
+        uint64_t t[8];
 #pragma unroll
-      for (int q = 0; q < 8; q++) t[q] = pp[(int64_t)(j + q) * cells];
+        for (int q = 0; q < 8; q++) t[q] = raw[at + (int64_t)(j + q) * cells];
 #pragma unroll
-      for (int q = 0; q < 8; q++) s += t[q];
+        for (int q = 0; q < 8; q++) add(t[q]);
+      }
+      for (; j < m; j++) add(raw[at + (int64_t)j * cells]);
+      const bool neg = (int64_t)hi < 0;
+      if (neg) {
+        lo = ~lo + 1;
+        hi = ~hi + (lo == 0 ? 1u : 0u);
+      }
+      s = ((double)hi * 18446744073709551616.0 + (double)lo) * scale;
+      if (neg) s = -s;
+    } else {
+      // eight loads in flight per thread; the additions keep the order j = 0, 1, ...
+      const double *pp = partials + at;
+      s = 0;
+      uint32_t j = 0;
+      for (; j + 8 <= m; j += 8) {
+        double t[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) t[q] = pp[(int64_t)(j + q) * cells];
+#pragma unroll
+        for (int q = 0; q < 8; q++) s += t[q];
+      }
+      for (; j < m; j++) s += pp[(int64_t)j * cells];
     }
-    for (; j < m; j++) s += pp[(int64_t)j * cells];
     const int64_t row = c / P.HZ;
     const int lz = (int)(c - row * P.HZ);
     Real *out = lz < P.kg.TZ ? tiles + (int64_t)k * P.cm + row * P.kg.TZ + lz : tiles + P.zoff + (int64_t)k * P.cz + row;
@@ -363,10 +408,10 @@ void deposit_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass) {
   {
     Stage st(p, "reduce_partials");
     if (p.prec == 4)
-      hipLaunchKernelGGL(k_reduce_partials<float>, dim3(1024), dim3(256), 0, s, nheavy, heavy, item_start, part_start,
-                         partials, P, p.tiles.as<float>());
+      hipLaunchKernelGGL((k_reduce_partials<float, NM>), dim3(1024), dim3(256), 0, s, nheavy, heavy, item_start,
+                         part_start, partials, P, p.tiles.as<float>());
     else
-      hipLaunchKernelGGL(k_reduce_partials<double>, dim3(1024), dim3(256), 0, s, nheavy, heavy, item_start,
+      hipLaunchKernelGGL((k_reduce_partials<double, NM>), dim3(1024), dim3(256), 0, s, nheavy, heavy, item_start,
                          part_start, partials, P, p.tiles.as<double>());
     GENPK_LAUNCH_CHECK();
   }

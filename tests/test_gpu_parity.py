@@ -216,6 +216,28 @@ def test_deposit_masses_fixed_point_is_deterministic(gpu):
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("with_masses", [False, True])
+def test_deposit_split_tiles_independent_of_particle_order(gpu, with_masses, prec):
+    """A tile of more than 2^17 particles is split into work items whose
+    partial tiles are summed exactly (fixed point), so the grid is
+    bit-identical whatever the particles' order, which decides the items'
+    membership (here a clump of 3e5 particles in a few cells, whose sums pass
+    2^53 in fixed point)."""
+    import torch
+    announce("deposit split tiles order", with_masses, prec)
+    d, box, n = 64, 64.0, 400000
+    rng = np.random.default_rng(19)
+    pos = np.concatenate([rng.normal(20.3, 0.4, (300000, 3)), rng.random((n - 300000, 3)) * box]).astype(np.float32)
+    masses = (rng.random(n) + 0.5).astype(np.float32) if with_masses else None
+    grids = []
+    for order in (np.arange(n), rng.permutation(n)):
+        p = genpk_amd.Plan(d, prec)
+        p.deposit(to_dev(torch, pos[order]), box, None if masses is None else to_dev(torch, masses[order]), 1.0)
+        grids.append(p.export_density().cpu().numpy())
+    assert np.array_equal(grids[0], grids[1])
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("with_masses", [False, True])
 def test_deposit_wide_key_vs_oracle(gpu, with_masses, prec):
     """d = 1280: 80 x 40 x 40 = 128000 tiles, a 17-bit key whose 9-bit low
     digit takes the second sort pass's histogram from the records (not from
