@@ -133,17 +133,19 @@ def test_adjoint_policies_agree(gpu):
     assert _close(r2[2], r3[2], 1e-10)[0] and _close(r2[3], r3[3], 1e-10)[0]
 
 
-def test_adjoint_x_1024_matches_unfused(gpu):
-    """d = 1024, FP32, where the adjoint x pass runs its first stage from the
-    loaded registers and alternates two LDS buffers: gradients with
-    recompute "always" (fused forward FFT -> weight -> inverse FFT per column
-    block) equal those of "never" (the weight applied to the kept spectrum,
-    then the unfused inverse passes) to FP32 rounding (2e-4 of the largest
-    component)."""
+@pytest.mark.parametrize("d,n", [(128, 1 << 18), (1024, 1 << 21)])
+def test_adjoint_x_fused_matches_unfused(gpu, d, n):
+    """d = 1024 and 128 (8- and 64-column blocks), FP32, where the adjoint x
+    pass runs its first stage from the loaded registers, alternates two LDS
+    buffers and reads the weight's per-bin factors, window factors and bin
+    thresholds from LDS: gradients with recompute "always" (fused forward FFT
+    -> weight -> inverse FFT per column block) equal those of "never" (the
+    weight applied to the kept spectrum, then the unfused inverse passes) to
+    FP32 rounding (2e-4 of the largest component)."""
     import torch
     from genpk_amd.autograd import chi2_loss
-    d, box, n = 1024, 1000.0, 1 << 21
-    announce("adjoint_x_1024", f"n={n}")
+    box = 1000.0 * d / 1024
+    announce("adjoint_x_fused", f"d={d} n={n}")
     pos = synth.clumped(n, box, d, n_clumps=256, seed=21).astype(np.float32)
     m = (np.random.default_rng(21).random(n) + 0.5).astype(np.float32)
     tp = torch.from_numpy(pos).cuda()
