@@ -332,12 +332,20 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather_tiles(const T *__rest
 // Gradients back to the particles' input order: particle j sits at sorted
 // position perm2[perm1[j]] (perm1 alone after a one-pass sort).  Reads of
 // perm1 and writes are coalesced; the two lookups are random reads, which
-// cost far less than scattered partial-line writes.
+// cost far less than scattered partial-line writes.  Each XCD (workgroup id
+// mod 8) takes runs of UNP_RUN consecutive blocks, the eight XCDs' runs side
+// by side: both partition passes are stable, so a run's particles of one
+// bucket sit together in perm2 and its lines are fetched by one L2, while
+// all XCDs stay within one window of the particles.  The grid is a multiple
+// of 8 * UNP_RUN.
+constexpr int UNP_RUN = 256;
 template <int RWO, int TWO>
 __global__ void k_unpermute(const float *__restrict__ rec, const uint32_t *__restrict__ perm1,
                             const uint32_t *__restrict__ perm2, int64_t n, float *__restrict__ gpos,
                             float *__restrict__ gmass) {
-  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+  const uint32_t x = blockIdx.x % NUM_XCD, slot = blockIdx.x / NUM_XCD;
+  const int64_t lb = ((int64_t)(slot / UNP_RUN) * NUM_XCD + x) * UNP_RUN + slot % UNP_RUN;
+  for (int64_t j = lb * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
     int64_t at = perm1[j];
     if (TWO) at = perm2[at];
     const float *r = rec + at * RWO;
@@ -496,7 +504,7 @@ void adjoint_gather(Plan &p, double boxsize, int64_t n, const float *pos, const 
     }
     {
       Stage st(p, "adjoint_unpermute");
-      const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(m, 256), 1 << 22);
+      const unsigned grid = (unsigned)(ceil_div(std::min<int64_t>(ceil_div(m, 256), 1 << 22), 8 * UNP_RUN) * 8 * UNP_RUN);
       const bool two = o.perm2 != nullptr;
       if (gm && two)
         hipLaunchKernelGGL((k_unpermute<4, 1>), dim3(grid), dim3(256), 0, p.stream, out, o.perm1, o.perm2, m, gp, gmp);
