@@ -34,6 +34,7 @@ namespace {
 
 constexpr int ADJ_THREADS = 1024;  // as the transform's column passes
 constexpr int GATHER_THREADS = 512;
+constexpr int GATHER_RUN = 32;
 template <typename T> struct AdjCfg;
 template <> struct AdjCfg<float> { static constexpr int MAXE = 8; };
 template <> struct AdjCfg<double> { static constexpr int MAXE = 4; };
@@ -245,8 +246,14 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather_tiles(const T *__rest
   constexpr int RWO = 3 + GM;
   extern __shared__ unsigned char lds_raw[];
   T *g = reinterpret_cast<T *>(lds_raw);
+  // items in runs of GATHER_RUN per XCD (workgroup id mod 8), the XCDs' runs
+  // side by side: consecutive tiles along z share the lines at their z edges
+  // in G, which one L2 then serves to both (the grid is a multiple of
+  // 8 * GATHER_RUN; ids past the last item return)
+  const uint32_t xcd = blockIdx.x % NUM_XCD, slot = blockIdx.x / NUM_XCD;
+  const uint32_t item = ((slot / GATHER_RUN) * NUM_XCD + xcd) * GATHER_RUN + slot % GATHER_RUN;
   ItemRange ir;
-  if (!item_range(item_tile, item_start, tile_start, P.ks, P.max_items, blockIdx.x, ir)) return;
+  if (!item_range(item_tile, item_start, tile_start, P.ks, P.max_items, item, ir)) return;
   if (ir.pb >= ir.pe) return;
   const int64_t k = ir.k;
   const int ntz = P.kg.ntz, nty = P.kg.nty, d = P.kg.d;
@@ -413,7 +420,8 @@ void gather_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass, fl
   P.mass = mass;
   P.fhz = FastDiv((uint32_t)tg.HZ);
   P.fhy = FastDiv((uint32_t)tg.HY);
-  hipLaunchKernelGGL((k_gather_tiles<T, NM, GM>), dim3((unsigned)o.max_items), dim3(GATHER_THREADS),
+  const int64_t grid = ceil_div(o.max_items, 8 * GATHER_RUN) * 8 * GATHER_RUN;
+  hipLaunchKernelGGL((k_gather_tiles<T, NM, GM>), dim3((unsigned)grid), dim3(GATHER_THREADS),
                      (size_t)tg.cells * sizeof(T), p.stream, p.field.as<const T>(), o.rec, o.tile_start, o.item_start,
                      o.item_tile, P, out);
   GENPK_LAUNCH_CHECK();
