@@ -50,6 +50,7 @@ class Plan:
         self.dims = int(dims)
         self.prec = _precision(precision)
         self.generation = 0  # bumped by every call that changes the plan's content
+        self._kept = None    # _particles_key of the kept tile order
         self.device = torch.device("cuda", torch.cuda.current_device() if device is None else device)
         self._h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
@@ -105,15 +106,42 @@ class Plan:
         self.generation += 1
         check(lib().genpk_clear(self._h))
 
+    @staticmethod
+    def _particles_key(pos, boxsize, masses, mass):
+        """Identity and in-place version of the particle tensors (the library
+        compares pointers only)."""
+        mk = None if masses is None else (masses.data_ptr(), masses._version)
+        return (pos.data_ptr(), pos._version, pos.shape[0], float(boxsize), mk, float(mass))
+
     def deposit(self, pos, boxsize, masses=None, mass=1.0, retain=False):
         """Accumulating CIC deposit (fieldize.cpp:46-114).  retain: keep the
         particles' tile order for a following adjoint_gather of the same
-        tensors."""
+        tensors, or for redeposit()."""
         self._check_particles(pos, masses)
         self._sync_stream()
         self.generation += 1
         fn = lib().genpk_deposit_retain if retain else lib().genpk_deposit
+        self._kept = None
         check(fn(self._h, float(boxsize), pos.shape[0], _ptr(pos), _ptr(masses), float(mass)))
+        if retain and pos.shape[0] <= (1 << 31):
+            self._kept = self._particles_key(pos, boxsize, masses, mass)
+
+    def keeps_order_of(self, pos, boxsize, masses=None, mass=1.0):
+        """Whether the kept tile order is that of these tensors, unchanged
+        since it was made."""
+        return self._kept is not None and self._kept == self._particles_key(pos, boxsize, masses, mass)
+
+    def redeposit(self):
+        """Deposit again the particles of the kept tile order (from its
+        sorted copy), skipping the partition."""
+        assert self._kept is not None, "no kept tile order: deposit with retain=True first"
+        self._sync_stream()
+        self.generation += 1
+        check(lib().genpk_redeposit(self._h))
+
+    def drop_order(self):
+        self._kept = None
+        check(lib().genpk_drop_order(self._h))
 
     def export_density(self, out=None, rowlen=None, accumulate=False):
         """Real-space density as a padded grid [d][d][rowlen] of the plan's precision."""
@@ -215,13 +243,19 @@ class Plan:
         return self.density()
 
     def adjoint_gather(self, pos, boxsize, masses=None, mass=1.0, want_mass_grad=True):
+        """dL/dpos (and dL/dm) of the particles from dL/drho in the plan.  A
+        kept tile order of these tensors is reused only if they were not
+        modified in place since it was made; the order used is kept."""
         self._check_particles(pos, masses)
         t = self.torch
         gpos = t.empty_like(pos)
         gm = t.empty(pos.shape[0], dtype=t.float32, device=self.device) if want_mass_grad else None
         self._sync_stream()
+        if not self.keeps_order_of(pos, boxsize, masses, mass):
+            self.drop_order()
         check(lib().genpk_adjoint_gather(self._h, float(boxsize), pos.shape[0], _ptr(pos), _ptr(masses), float(mass),
                                          _ptr(gpos), _ptr(gm)))
+        self._kept = self._particles_key(pos, boxsize, masses, mass) if pos.shape[0] <= (1 << 31) else None
         return gpos, gm
 
 

@@ -28,6 +28,8 @@ SIGNATURES = {
     "genpk_clear": (_c_int, [_c_vp]),
     "genpk_deposit": (_c_int, [_c_vp, _c_dbl, _c_i64, _c_vp, _c_vp, _c_dbl]),
     "genpk_deposit_retain": (_c_int, [_c_vp, _c_dbl, _c_i64, _c_vp, _c_vp, _c_dbl]),
+    "genpk_redeposit": (_c_int, [_c_vp]),
+    "genpk_drop_order": (_c_int, [_c_vp]),
     "genpk_export_density": (_c_int, [_c_vp, _c_vp, _c_i64, _c_int]),
     "genpk_import_density": (_c_int, [_c_vp, _c_vp]),
     "genpk_import_spectrum": (_c_int, [_c_vp, _c_vp]),

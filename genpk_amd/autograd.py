@@ -33,7 +33,10 @@ class PowerSpectrumFunction(torch.autograd.Function):
                 z and y passes, plus the particles' tile order); the backward
                 starts from it if the plan has not been used since, and
                 otherwise recomputes deposit + transform.
-      "always": the backward recomputes deposit + transform.
+      "always": the backward recomputes deposit + transform; the deposit
+                runs from the particles' tile order the forward kept (their
+                sorted copy, particle data), so the partition is not redone
+                while the plan still holds it.
       "never":  the forward keeps a copy of the spectrum (d*d*(d/2+1)
                 complex values) for the backward.
     """
@@ -43,7 +46,7 @@ class PowerSpectrumFunction(torch.autograd.Function):
         M = float(total_mass)
         mdet = None if masses is None else masses.detach()
         plan.clear()
-        plan.deposit(pos.detach(), boxsize, mdet, mass, retain=recompute != "always")
+        plan.deposit(pos.detach(), boxsize, mdet, mass, retain=True)
         spec = None
         if recompute == "never":
             spec = plan.r2c().clone()
@@ -65,9 +68,14 @@ class PowerSpectrumFunction(torch.autograd.Function):
         if ctx.spec is not None:
             plan.import_spectrum(ctx.spec)
         elif ctx.recompute == "always" or plan.generation != ctx.generation:
-            # recompute: the adjoint redoes the forward transform of this deposit
+            # recompute the forward density (the adjoint then redoes its
+            # transform): from the tile order the forward kept when it is
+            # still this call's, else by a full deposit
             plan.clear()
-            plan.deposit(pos.detach(), ctx.boxsize, mdet, ctx.mass, retain=True)
+            if plan.keeps_order_of(pos.detach(), ctx.boxsize, mdet, ctx.mass):
+                plan.redeposit()
+            else:
+                plan.deposit(pos.detach(), ctx.boxsize, mdet, ctx.mass, retain=True)
         lam = gP.detach().to(torch.float64).contiguous()
         plan.adjoint_density(lam, ctx.M, ctx.nrbins)
         want_m = ctx.needs_input_grad[1]

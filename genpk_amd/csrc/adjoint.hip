@@ -489,6 +489,8 @@ void adjoint_gather(Plan &p, double boxsize, int64_t n, const float *pos, const 
       p.order.masses = masses;
       p.order.n = n;
       p.order.boxsize = boxsize;
+      p.order.mass = mass;
+      p.order.every_tile = false;
       p.order.to = o;
     }
     float *gp = grad_pos + 3 * done;

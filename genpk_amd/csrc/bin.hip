@@ -520,6 +520,15 @@ BinTables &bin_tables(Plan &p, int nrbins) {
   return ref;
 }
 
+const BinTables &built_bin_tables(const Plan &p, int nrbins) {
+  GENPK_REQUIRE(nrbins >= 1 && nrbins <= 8192, "nrbins must be in [1, 8192]");
+  auto it = p.bins.find(nrbins);
+  GENPK_REQUIRE(it != p.bins.end() && it->second->counts_ready && p.wtab.ptr,
+                "the bin tables of this nrbins are not built on the plan: call genpk_power or genpk_bin with it once "
+                "first");
+  return *it->second;
+}
+
 // Largest column block for the fused x-pass + bin kernel, 0 if none fits.
 static int fused_block(int prec, int64_t d, int64_t hz, int nf, int ns, int nrbins) {
   const int csize = 2 * prec;
@@ -589,36 +598,69 @@ static void launch_stored(Plan &a, Plan *b, const BinTables &bt, double *part, i
   GENPK_LAUNCH_CHECK();
 }
 
+// Launch shape of the fused x pass + bin: column block B, ping-pong buffers
+// (pp), the register first stage (reg1), the radix-2 last stage in the
+// binning (r2l), and the persistent grid G.
+struct FusedShape {
+  int B = 0;
+  bool pp = false, reg1 = false, r2l = false;
+  int G = 0;
+};
+
+static FusedShape fused_shape(const Plan &a, const Plan *b, int nrbins) {
+  const int nf = b ? 2 : 1, ns = b ? 3 : 1;
+  FusedShape s;
+  s.B = fused_block(a.prec, a.d, a.hz, nf, ns, nrbins);
+  GENPK_REQUIRE(s.B > 0, "fused x-pass does not fit in LDS for this size");
+  const int B = s.B;
+  // one field: ping-pong FFT buffers where the LDS has room for them
+  s.pp = !b && (a.prec == 4 ? fused_lds<float, 1>(a, B, nrbins, true)
+                            : fused_lds<double, 1>(a, B, nrbins, true)) <= (size_t)LDS_LIMIT;
+  s.reg1 = !b && (a.prec == 4 ? fused_reg1<float>(a, B, s.pp) : fused_reg1<double>(a, B, s.pp));
+  // FP32 with a last stage of radix 2: that stage runs in the binning
+  const RadixPlan xrp = make_radix_plan((int)a.d);
+  s.r2l = s.reg1 && a.prec == 4 && xrp.st[xrp.nst - 1].R == 2;
+  const bool pp = s.pp;
+  if (a.prec == 4)
+    s.G = b ? fused_grid<float, 2, false>(a, B, nrbins, false)
+            : s.r2l ? fused_grid<float, 1, true, true>(a, B, nrbins, pp)
+                    : s.reg1 ? fused_grid<float, 1, true>(a, B, nrbins, pp) : fused_grid<float, 1, false>(a, B, nrbins, pp);
+  else
+    s.G = b ? fused_grid<double, 2, false>(a, B, nrbins, false)
+            : s.reg1 ? fused_grid<double, 1, true>(a, B, nrbins, pp) : fused_grid<double, 1, false>(a, B, nrbins, pp);
+  return s;
+}
+
+int64_t fused_partial_doubles(const Plan &a, const Plan *b, int nrbins) {
+  return (int64_t)fused_shape(a, b, nrbins).G * (b ? 3 : 1) * nrbins;
+}
+
 void bin_spectra(Plan &a, Plan *b, int nrbins, bool fused_xpass, double *pow_aa, double *pow_bb, double *pow_ab,
-                 int *count, double *keffs, double ma, double mb) {
+                 int *count, double *keffs, double ma, double mb, double *scratch, size_t scratch_doubles) {
   if (b) {
     GENPK_REQUIRE(b->d == a.d && b->prec == a.prec, "cross spectrum needs plans of equal dims and precision");
     GENPK_REQUIRE(b->stream == a.stream, "cross spectrum needs plans on the same stream");
   }
-  const BinTables &bt = bin_tables(a, nrbins);
+  const BinTables &bt = scratch ? built_bin_tables(a, nrbins) : bin_tables(a, nrbins);
   const int nf = b ? 2 : 1, ns = b ? 3 : 1;
   int G = std::max(1, std::min<int>(2 * num_cus(), (int)std::min<int64_t>(a.d * a.d, 1 << 20)));
   double *part = nullptr;
+  // per-workgroup partial bins: the caller's scratch, or the plan's buffer
+  auto partials = [&](int64_t doubles) -> double * {
+    if (!scratch) return static_cast<double *>(a.bin_part.ensure((size_t)doubles * 8));
+    GENPK_REQUIRE((size_t)doubles <= scratch_doubles,
+                  "the partial bins (" + std::to_string(doubles) + " doubles: workgroups x spectra x nrbins) exceed "
+                  "the scratch of " + std::to_string(scratch_doubles) + " doubles: use fewer bins");
+    return scratch;
+  };
   if (fused_xpass) {
     GENPK_REQUIRE(a.fstate == FieldState::YDone && (!b || b->fstate == FieldState::YDone),
                   "fused binning needs fields after the z and y passes");
-    const int B = fused_block(a.prec, a.d, a.hz, nf, ns, nrbins);
-    GENPK_REQUIRE(B > 0, "fused x-pass does not fit in LDS for this size");
-    // one field: ping-pong FFT buffers where the LDS has room for them
-    const bool pp = !b && (a.prec == 4 ? fused_lds<float, 1>(a, B, nrbins, true)
-                                       : fused_lds<double, 1>(a, B, nrbins, true)) <= (size_t)LDS_LIMIT;
-    const bool reg1 = !b && (a.prec == 4 ? fused_reg1<float>(a, B, pp) : fused_reg1<double>(a, B, pp));
-    // FP32 with a last stage of radix 2: that stage runs in the binning
-    const RadixPlan xrp = make_radix_plan((int)a.d);
-    const bool r2l = reg1 && a.prec == 4 && xrp.st[xrp.nst - 1].R == 2;
-    if (a.prec == 4)
-      G = b ? fused_grid<float, 2, false>(a, B, nrbins, false)
-            : r2l ? fused_grid<float, 1, true, true>(a, B, nrbins, pp)
-                  : reg1 ? fused_grid<float, 1, true>(a, B, nrbins, pp) : fused_grid<float, 1, false>(a, B, nrbins, pp);
-    else
-      G = b ? fused_grid<double, 2, false>(a, B, nrbins, false)
-            : reg1 ? fused_grid<double, 1, true>(a, B, nrbins, pp) : fused_grid<double, 1, false>(a, B, nrbins, pp);
-    part = static_cast<double *>(a.bin_part.ensure((size_t)G * ns * nrbins * 8));
+    const FusedShape fs = fused_shape(a, b, nrbins);
+    const int B = fs.B;
+    const bool pp = fs.pp, reg1 = fs.reg1, r2l = fs.r2l;
+    G = fs.G;
+    part = partials((int64_t)G * ns * nrbins);
     if (a.prec == 4) {
       if (b) launch_fused<float, 2, false>(a, b, bt, B, false, false, part, G);
       else if (r2l) launch_fused<float, 1, true, true>(a, b, bt, B, false, pp, part, G);
@@ -635,7 +677,7 @@ void bin_spectra(Plan &a, Plan *b, int nrbins, bool fused_xpass, double *pow_aa,
   } else {
     GENPK_REQUIRE(a.fstate == FieldState::Spectrum && (!b || b->fstate == FieldState::Spectrum),
                   "binning needs transformed fields");
-    part = static_cast<double *>(a.bin_part.ensure((size_t)G * ns * nrbins * 8));
+    part = partials((int64_t)G * ns * nrbins);
     if (a.prec == 4) {
       if (b) launch_stored<float, 2>(a, b, bt, part, G); else launch_stored<float, 1>(a, b, bt, part, G);
     } else {

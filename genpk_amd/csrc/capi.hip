@@ -73,12 +73,6 @@ Plan::Plan(int64_t dims, int precision, hipStream_t s) {
   if (fftable) fft_tables_init(*this);
 }
 
-// The field holds no density (None), a density (Real), or the forward state
-// left by the fused power (YDone), which counts as no density.
-static bool density_state(const Plan &p) {
-  return p.fstate == FieldState::None || p.fstate == FieldState::Real || p.fstate == FieldState::YDone;
-}
-
 bool plan_fft_ok(const Plan &p) {
   return (p.d % 2 == 0) && p.d >= 2 && fft_length_ok(p.d) && fft_length_ok(p.d / 2);
 }
@@ -107,10 +101,6 @@ static Plan &compat_plan(std::map<int64_t, std::unique_ptr<Plan>> &m, int64_t di
 }  // namespace genpk
 
 using namespace genpk;
-
-struct genpk_plan {
-  std::unique_ptr<Plan> p;
-};
 
 extern "C" {
 
@@ -188,6 +178,22 @@ int genpk_deposit_retain(genpk_plan *plan, double boxsize, int64_t n, const floa
     Plan &p = *plan->p;
     GENPK_REQUIRE(density_state(p), "deposit after a transform: clear the plan first");
     deposit(p, boxsize, n, pos, masses, mass, true);
+  });
+}
+
+int genpk_redeposit(genpk_plan *plan) {
+  return guarded([&] {
+    GENPK_REQUIRE(plan && plan->p, "null plan");
+    Plan &p = *plan->p;
+    GENPK_REQUIRE(density_state(p), "deposit after a transform: clear the plan first");
+    redeposit(p);
+  });
+}
+
+int genpk_drop_order(genpk_plan *plan) {
+  return guarded([&] {
+    GENPK_REQUIRE(plan && plan->p, "null plan");
+    plan->p->order.valid = false;
   });
 }
 

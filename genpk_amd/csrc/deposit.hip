@@ -56,6 +56,7 @@ struct DepParams {
   double mass;         // constant mass (u64 path)
   double bias;         // 2^52 (to_fixed)
   const uint32_t *mrange;  // per-particle masses: the sort's |mass| range (float bits)
+  uint32_t *nonfinite;     // set to 1 when a particle's position is not finite
   int store;           // 1: write tiles, 0: add to tiles
 };
 
@@ -149,6 +150,11 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
         const int lz = cell_of(r[u][2], P.kg.units, P.kg.d, fz) - oz;
         if ((unsigned)lx >= (unsigned)TX || (unsigned)ly >= (unsigned)TY || (unsigned)lz >= (unsigned)TZ)
           continue;  // cannot happen for records of this tile; guards the LDS bounds
+        if (!(fx + fy + fz <= 3.0)) {
+          // a non-finite position: k_poison sets its eight cells to NaN
+          *P.nonfinite = 1u;
+          continue;
+        }
         const double gx = 1.0 - fx, gy = 1.0 - fy, gz = 1.0 - fz;
         const double wxy[4] = {gx * gy, fx * gy, gx * fy, fx * fy};
         const double gzs = gz * FIXED_ONE, fzs = fz * FIXED_ONE;  // exact: a power-of-two scale
@@ -287,6 +293,33 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const uint32_t *__restr
   }
 }
 
+// Particles with a non-finite position (found by k_deposit, which skips
+// them) make the reference's eight CIC weights non-finite (fieldize.cpp:
+// 67-84: the fractional part is NaN or infinite), so their eight cells end
+// up NaN or infinite there.  Here those cells are set to NaN in the tile
+// store, at the cells cell_of gives (those of the reference on x86-64).
+// The records are scanned only when k_deposit raised the flag.  A cell's
+// own store location is written (never a halo), so the order against the
+// halo folds does not matter, and later deposits keep it NaN.
+template <typename Real>
+__global__ void __launch_bounds__(256) k_poison(const float *__restrict__ rec, int rw, int64_t n, KeyGeom kg,
+                                                FoldGeom fg, const uint32_t *__restrict__ nonfinite,
+                                                Real *__restrict__ tiles) {
+  if (*nonfinite == 0u) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float *r = rec + i * rw;
+    double fx, fy, fz;
+    const int cx = cell_of(r[0], kg.units, kg.d, fx);
+    const int cy = cell_of(r[1], kg.units, kg.d, fy);
+    const int cz = cell_of(r[2], kg.units, kg.d, fz);
+    if (fx + fy + fz <= 3.0) continue;
+    const int nx = cx + 1 == kg.d ? 0 : cx + 1, ny = cy + 1 == kg.d ? 0 : cy + 1, nz = cz + 1 == kg.d ? 0 : cz + 1;
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+      tiles[cell_at(fg, (c & 1) ? nx : cx, (c & 2) ? ny : cy, (c & 4) ? nz : cz)] = (Real)NAN;
+  }
+}
+
 // In-place halo folds along y, then x.  fold_y: one workgroup per tile t adds
 // the y-halo rows (local y = width of the previous tile along y) of that
 // tile into t's rows at local y = 0, for every local x including t's x-halo
@@ -385,7 +418,9 @@ void deposit_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass) {
   P.mass = mass;
   P.bias = 4503599627370496.0;
   P.mrange = o.mrange;
+  P.nonfinite = static_cast<uint32_t *>(p.nonfinite.ensure(64));
   P.store = store;
+  fill_zero(P.nonfinite, 4, s);
   const size_t lds = (size_t)tg.cells * 8;
   {
     Stage st(p, "deposit_tiles");
@@ -413,6 +448,18 @@ void deposit_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass) {
     else
       hipLaunchKernelGGL((k_reduce_partials<double, NM>), dim3(1024), dim3(256), 0, s, nheavy, heavy, item_start,
                          part_start, partials, P, p.tiles.as<double>());
+    GENPK_LAUNCH_CHECK();
+  }
+  {
+    // returns at once unless a position was not finite
+    const FoldGeom fg = fold_geom(p);
+    const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(n, 256), 2048);
+    if (p.prec == 4)
+      hipLaunchKernelGGL(k_poison<float>, dim3(grid), dim3(256), 0, s, sorted, o.rw, n, kg, fg, P.nonfinite,
+                         p.tiles.as<float>());
+    else
+      hipLaunchKernelGGL(k_poison<double>, dim3(grid), dim3(256), 0, s, sorted, o.rw, n, kg, fg, P.nonfinite,
+                         p.tiles.as<double>());
     GENPK_LAUNCH_CHECK();
   }
   {
@@ -483,7 +530,8 @@ void deposit(Plan &p, double boxsize, int64_t n, const float *pos, const float *
   do {
     const int64_t m = std::min(n - done, cap);
     const float *mp = masses ? masses + done : nullptr;
-    const TileOrder o = sort_by_tile(p, kg, m, pos + 3 * done, mp, keep, !p.tiles_live);
+    const bool live_before = p.tiles_live;
+    const TileOrder o = sort_by_tile(p, kg, m, pos + 3 * done, mp, keep, !live_before);
     if (masses)
       deposit_impl<1, 4>(p, kg, o, mass);
     else
@@ -495,10 +543,29 @@ void deposit(Plan &p, double boxsize, int64_t n, const float *pos, const float *
       p.order.masses = masses;
       p.order.n = n;
       p.order.boxsize = boxsize;
+      p.order.mass = mass;
+      p.order.every_tile = !live_before;
       p.order.to = o;
     }
     done += m;
   } while (done < n);
+}
+
+void redeposit(Plan &p) {
+  GENPK_REQUIRE(p.order.valid, "no kept tile order: deposit with retain first");
+  const RetainedOrder &r = p.order;
+  p.tiles.ensure((size_t)p.tg.K * (p.tg.cm + p.tg.cz) * p.prec);
+  if (!p.tiles_live && !r.every_tile) {
+    // the kept items skip empty tiles: start from a zeroed store and accumulate
+    fill_zero(p.tiles.ptr, (size_t)p.tg.K * (p.tg.cm + p.tg.cz) * p.prec, p.stream);
+    p.tiles_live = true;
+  }
+  const KeyGeom kg = key_geom(p, r.boxsize);
+  if (r.to.has_mass)
+    deposit_impl<1, 4>(p, kg, r.to, r.mass);
+  else
+    deposit_impl<0, 3>(p, kg, r.to, r.mass);
+  p.tiles_live = true;
 }
 
 void fold_tiles(Plan &p, void *out, int64_t out_rowlen, bool accumulate) {

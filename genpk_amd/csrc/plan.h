@@ -101,12 +101,16 @@ struct TileOrder {
 };
 
 // Tile order kept from the last deposit made with retain = true, for the
-// adjoint gather of the same particle arrays.
+// adjoint gather of the same particle arrays and for depositing them again
+// (redeposit).  every_tile: the order's work items cover every tile (it was
+// made by a deposit into an empty plan).
 struct RetainedOrder {
   bool valid = false;
   const float *pos = nullptr, *masses = nullptr;
   int64_t n = 0;
   double boxsize = 0;
+  double mass = 1;
+  bool every_tile = false;
   TileOrder to;
 };
 
@@ -156,6 +160,7 @@ struct Plan {
   DevBuf permA, permB;      // sort permutations kept for the adjoint gather
   DevBuf dig2;              // low tile-key digit of each pass-1 output record (one byte)
   DevBuf mrange;            // largest and smallest nonzero |mass| of the last sort (float bits)
+  DevBuf nonfinite;         // flag of the last deposit: a particle position is not finite
   DevBuf scan_tmp;
   RetainedOrder order;
 
@@ -190,6 +195,12 @@ struct Stage {
   }
 };
 
+// The field holds no density (None), a density (Real), or the forward state
+// left by the fused power (YDone), which counts as no density.
+inline bool density_state(const Plan &p) {
+  return p.fstate == FieldState::None || p.fstate == FieldState::Real || p.fstate == FieldState::YDone;
+}
+
 struct FoldGeom;
 FoldGeom fold_geom(const Plan &p);
 // capi.hip: dims the FFT supports (even, dims/2 = 2^a 3^b 5^c)
@@ -198,6 +209,9 @@ bool plan_fft_ok(const Plan &p);
 // deposit.hip
 void deposit(Plan &p, double boxsize, int64_t n, const float *pos, const float *masses, double mass,
              bool retain = false);
+// Deposit again the particles of the kept tile order (p.order) from its
+// sorted records, skipping the partition.
+void redeposit(Plan &p);
 // Compute units of the current device.
 int num_cus();
 void fold_tiles(Plan &p, void *out, int64_t out_rowlen, bool accumulate);
@@ -215,11 +229,27 @@ void r2c(Plan &p);
 void c2r(Plan &p);
 // bin.hip
 BinTables &bin_tables(Plan &p, int nrbins);
+// The bin tables of nrbins if an earlier call built them, else an error
+// (builds nothing, allocates nothing).
+const BinTables &built_bin_tables(const Plan &p, int nrbins);
+// Bin the spectra (fused_xpass: the fields after the z and y passes, the x
+// pass run in the binning).  scratch: caller memory for the per-workgroup
+// partial bins (scratch_doubles long) instead of the plan's buffer; with
+// it the bin tables must already be built and nothing is allocated.
 void bin_spectra(Plan &a, Plan *b, int nrbins, bool fused_xpass, double *pow_aa, double *pow_bb,
-                 double *pow_ab, int *count, double *keffs, double ma, double mb);
+                 double *pow_ab, int *count, double *keffs, double ma, double mb, double *scratch = nullptr,
+                 size_t scratch_doubles = 0);
+// Doubles of per-workgroup partial bins the fused binning of a (and b) with
+// nrbins bins writes.
+int64_t fused_partial_doubles(const Plan &a, const Plan *b, int nrbins);
 // adjoint.hip
 void adjoint_density(Plan &p, int nrbins, const double *lambda, double m);
 void adjoint_gather(Plan &p, double boxsize, int64_t n, const float *pos, const float *masses, double mass,
                     float *grad_pos, float *grad_mass);
 
 }  // namespace genpk
+
+// The C ABI's opaque plan handle (include/genpk_amd.h).
+struct genpk_plan {
+  std::unique_ptr<genpk::Plan> p;
+};

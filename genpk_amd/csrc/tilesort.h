@@ -17,13 +17,16 @@ KeyGeom key_geom(const Plan &p, double boxsize);
 
 // Floor cell of one coordinate, wrapped into [0, d), and the fractional part
 // (fieldize.cpp:66-75).  The floor is converted to a 32-bit integer as the
-// reference does; positions more than one box outside [0, box) take the
-// general modulo.
+// reference does; a floor outside the int range, or a non-finite position,
+// converts to INT_MIN, which is what that conversion gives on x86-64 (the
+// "integer indefinite" of cvttsd2si), so such a particle lands on the same
+// cells as there (its fractional part is then not finite).  Positions more
+// than one box outside [0, box) take the general modulo.
 __device__ __forceinline__ int cell_of(float p, double units, int d, double &frac) {
   const double u = (double)p * units;
   const double f = floor(u);
   frac = u - f;
-  int c = (int)f;
+  int c = (f >= -2147483648.0 && f < 2147483648.0) ? (int)f : (int)0x80000000u;
   if ((unsigned)c >= (unsigned)d) {
     c %= d;
     if (c < 0) c += d;

@@ -11,6 +11,9 @@ builds the 1024^3-particle clustered workload directly on the GPU.
   ``rms_cells`` cells rms per component.
 * ``clumped``: half uniform, half in ``n_clumps`` Gaussian clumps with
   power-law (pdf ~ n^-1.9) member counts and radii U[0.5, 4] cells.
+
+The ``*_torch`` versions build the same recipes on the device (torch's
+generators and FFT are used for input generation only).
 """
 import hashlib
 
@@ -34,10 +37,10 @@ def gaussian_lattice(n_side, box, dims, sigma_cells, seed=0):
     return np.mod(p, box).astype(np.float32)
 
 
-def zeldovich_displacement(n_side, rms_cells, box, dims, seed=0):
-    """Displacement field [n_side^3, 3] (float64, box units)."""
-    rng = np.random.default_rng(seed)
-    w = rng.standard_normal((n_side, n_side, n_side))
+def zeldovich_displacement(n_side, rms_cells, box, dims, seed=0, noise=None):
+    """Displacement field [n_side^3, 3] (float64, box units) from seeded
+    white noise (or the given ``noise`` [n_side]^3)."""
+    w = np.random.default_rng(seed).standard_normal((n_side, n_side, n_side)) if noise is None else noise
     wk = np.fft.rfftn(w)
     k = np.fft.fftfreq(n_side) * n_side
     kz = np.fft.rfftfreq(n_side) * n_side
@@ -48,7 +51,7 @@ def zeldovich_displacement(n_side, rms_cells, box, dims, seed=0):
     for ka in (KX, KY, KZ):
         ph = 1j * ka / k2 * wk
         ph[0, 0, 0] = 0
-        out.append(np.fft.irfftn(ph, s=(n_side,) * 3).ravel())
+        out.append(np.fft.irfftn(ph, s=(n_side,) * 3, axes=(0, 1, 2)).ravel())
     psi = np.stack(out, axis=1)
     psi *= (rms_cells * box / dims) / np.sqrt(np.mean(psi ** 2))
     return psi
@@ -110,6 +113,56 @@ def clumped_torch(n, box, dims, n_clumps=100000, seed=4, r_min=0.5, r_max=4.0, d
         p = centres[o] + torch.randn((e - s, 3), generator=g, device=device, dtype=torch.float64) * radii[o, None]
         out[nu + s:nu + e] = torch.remainder(p, box).float()
     del owner
+    return out
+
+
+def zeldovich_displacement_torch(n_side, rms_cells, box, dims, seed=1, device="cuda"):
+    """``zeldovich_displacement`` computed on the device in FP64 (torch's
+    Philox white noise, torch.fft): psi_hat_a = i k_a / |k|^2 w_hat, zero at
+    k = 0, scaled to ``rms_cells`` cells (of a ``dims`` grid) rms per
+    component.  Returns float64 [n_side^3, 3] in box units."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    n = n_side
+    w = torch.randn((n, n, n), generator=g, dtype=torch.float64, device=device)
+    wk = torch.fft.rfftn(w)
+    del w
+    k = torch.fft.fftfreq(n, d=1.0 / n, dtype=torch.float64, device=device)
+    kz = torch.fft.rfftfreq(n, d=1.0 / n, dtype=torch.float64, device=device)
+    k2 = k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2
+    k2[0, 0, 0] = 1.0
+    base = wk / k2
+    base[0, 0, 0] = 0
+    del wk, k2
+    psi = torch.empty((n * n * n, 3), dtype=torch.float64, device=device)
+    shapes = ((n, 1, 1), (1, n, 1), (1, 1, n // 2 + 1))
+    for a, ka in enumerate((k, k, kz)):
+        ph = base * (1j * ka.reshape(shapes[a]))
+        psi[:, a] = torch.fft.irfftn(ph, s=(n, n, n)).reshape(-1)
+        del ph
+    psi *= (rms_cells * box / dims) / torch.sqrt(torch.mean(psi * psi))
+    return psi
+
+
+def zeldovich_torch(n_side, box, dims, rms_cells=1.0, seed=1, offset_cells=0.0, psi=None, device="cuda"):
+    """``zeldovich`` on the device: an n_side^3 lattice (offset by
+    ``offset_cells`` lattice spacings) displaced by psi (built from ``seed``
+    unless given), wrapped into [0, box), float32 [n_side^3, 3]."""
+    import torch
+    if psi is None:
+        psi = zeldovich_displacement_torch(n_side, rms_cells, box, dims, seed, device)
+    ar = (torch.arange(n_side, dtype=torch.float64, device=device) + offset_cells) * (box / n_side)
+    out = torch.empty((n_side ** 3, 3), dtype=torch.float32, device=device)
+    per = n_side * n_side  # one x-plane of the lattice at a time
+    yy, zz = torch.meshgrid(ar, ar, indexing="ij")
+    yz = torch.stack([yy.reshape(-1), zz.reshape(-1)], 1)
+    for i in range(n_side):
+        s = slice(i * per, (i + 1) * per)
+        p = psi[s].clone()
+        p[:, 0] += ar[i]
+        p[:, 1:] += yz
+        out[s] = torch.remainder(p, box).float()
     return out
 
 

@@ -70,6 +70,19 @@ int genpk_deposit(genpk_plan *plan, double boxsize, int64_t n, const float *pos,
 int genpk_deposit_retain(genpk_plan *plan, double boxsize, int64_t n, const float *pos, const float *masses,
                          double mass);
 
+/* Deposit again the particles whose tile order the last
+ * genpk_deposit_retain (or genpk_adjoint_gather) kept, from the kept sorted
+ * copy of their positions and masses (with the same boxsize and mass): the
+ * tile accumulation runs without the partition.  This is how an adjoint
+ * recomputes the forward density instead of storing the field.  Fails if no
+ * order is kept.  No counterpart in the reference. */
+int genpk_redeposit(genpk_plan *plan);
+
+/* Forget the kept tile order, so that the next genpk_adjoint_gather
+ * partitions the particles again (for callers whose particle arrays changed
+ * in place since the retaining deposit). */
+int genpk_drop_order(genpk_plan *plan);
+
 /* Write the real-space density into out, a padded grid [d][d][out_rowlen]
  * of the plan's precision; accumulate != 0 adds to out.  Pad columns
  * (z >= d) are left untouched. */

@@ -20,6 +20,7 @@
  * force on tiny inputs, closed forms and invariants (mass conservation,
  * sum of counts = d^3-1, translation = roll, linearity).
  */
+#include <limits.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -67,7 +68,12 @@ int orc_fieldize(double boxsize, int dims, double *out, int64_t n,
             int64_t i0[3], i1[3];
             for (int a = 0; a < 3; a++) {
                 const double u = (double)pos[3 * j + a] * units;
-                int fl = (int)floor(u);
+                /* fieldize.cpp:67 stores floor(x) into an int; on x86-64
+                 * that conversion (cvttsd2si) gives INT_MIN for a NaN, an
+                 * infinity or a value outside the int range, spelled out
+                 * here so the C code has no undefined conversion */
+                const double fd = floor(u);
+                int fl = (fd >= -2147483648.0 && fd < 2147483648.0) ? (int)fd : INT_MIN;
                 const double f = u - fl;
                 hi[a] = f;
                 lo[a] = 1.0 - f;

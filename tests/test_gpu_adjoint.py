@@ -284,3 +284,48 @@ def test_total_mass_gradient(gpu):
     (gM,) = torch.autograd.grad((P * w).sum(), (M,))
     expect = -2.0 * (P.detach() * w).sum().item() / M.item()
     assert abs(gM.item() - expect) <= 1e-12 * abs(expect)
+
+
+@pytest.mark.parametrize("d", [48, 512])
+def test_redeposit_and_stale_order(gpu, d):
+    """d = 48 (one-pass tile sort) and 512 (two passes).  (a) redeposit()
+    from the tile order a retaining deposit into an empty plan kept, and
+    from the order a gather made (work items only on occupied tiles, so the
+    store is zeroed first), gives that deposit's density bit for bit (fixed
+    point: the sums do not depend on which records form which item);
+    (b) positions changed in place after the retaining deposit: the gather
+    partitions them again instead of reusing the stale order, so the
+    gradients equal a fresh plan's on the same dL/drho."""
+    import torch
+    box = 100.0
+    announce("redeposit and stale order", f"d={d}")
+    pos, m = _instance(24, d, box, seed=17)
+    tp = torch.from_numpy(pos).cuda()
+    tm = torch.from_numpy(m).cuda()
+    a = genpk_amd.Plan(d, "f64")
+    a.clear()
+    a.deposit(tp, box, tm, retain=True)
+    ref = a.export_density().clone()
+    a.clear()
+    a.redeposit()
+    assert torch.equal(a.export_density(), ref)
+    # an order made by the gather of an imported dL/drho
+    G = torch.randn(ref.shape, dtype=torch.float64, device="cuda", generator=torch.Generator(device="cuda").manual_seed(3))
+    b = genpk_amd.Plan(d, "f64")
+    b.import_density(G)
+    b.adjoint_gather(tp, box, tm)
+    b.clear()
+    b.redeposit()
+    assert torch.equal(b.export_density(), ref)
+    # (b) stale order
+    a.clear()
+    a.deposit(tp, box, tm, retain=True)
+    tp.add_(0.37 * box / d)
+    tp.remainder_(box)
+    assert not a.keeps_order_of(tp, box, tm)
+    a.import_density(G)
+    g_a = a.adjoint_gather(tp, box, tm)
+    c = genpk_amd.Plan(d, "f64")
+    c.import_density(G)
+    g_c = c.adjoint_gather(tp, box, tm)
+    assert torch.equal(g_a[0], g_c[0]) and torch.equal(g_a[1], g_c[1])

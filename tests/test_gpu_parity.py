@@ -283,6 +283,61 @@ def test_deposit_heavy_tile_split(gpu):
         assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), prec
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("kind", ["signed", "net_negative"])
+def test_deposit_heavy_tile_split_signed_masses(gpu, kind, prec):
+    """A tile of 3e5 particles (split into work items whose partial tiles
+    are summed as signed 128-bit integers) with per-particle masses: about
+    10% negative ("signed"), or mostly negative so that the heavy tile's
+    cell sums are net negative ("net_negative"); against the oracle."""
+    import torch
+    announce("deposit heavy tile signed masses", kind, prec)
+    rng = np.random.default_rng(6 if kind == "signed" else 7)
+    d, box, n = 64, 64.0, 400000
+    pos = np.concatenate([rng.normal(20.3, 0.4, (300000, 3)), rng.random((n - 300000, 3)) * box]).astype(np.float32)
+    mag = (rng.random(n) * 900 + 1.5).astype(np.float32)
+    neg = rng.random(n) < (0.1 if kind == "signed" else 0.8)
+    masses = np.where(neg, -mag, mag).astype(np.float32)
+    got, ref = deposit_case(torch, d, box, pos, masses, 1.0, prec)
+    if kind == "net_negative":
+        assert ref.min() < 0 and ref.sum() < 0
+    tol = 1e-9 if prec == "f64" else 2e-7
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("masses_kind", [None, "fixed", "wide"])
+def test_deposit_nonfinite_positions(gpu, masses_kind, prec):
+    """A NaN, a +inf and a -inf coordinate (three particles, three axes)
+    make each particle's eight cells non-finite, as the reference's weights
+    are (fieldize.cpp:67-84), at the cells the oracle puts them (the floor
+    of a non-finite value converts to INT_MIN on x86-64); every other cell
+    matches the oracle, and P(k) is NaN in every occupied bin.  With masses
+    in the fixed-point range and beyond it (FP64 accumulation)."""
+    import torch
+    announce("deposit non-finite positions", masses_kind, prec)
+    d, box, n = 24, 10.0, 5000
+    rng = np.random.default_rng(23)
+    pos = (rng.random((n, 3)) * box).astype(np.float32)
+    pos[10, 0], pos[20, 1], pos[30, 2] = np.nan, np.inf, -np.inf
+    masses = None
+    if masses_kind == "fixed":
+        masses = (rng.random(n) + 0.5).astype(np.float32)
+    elif masses_kind == "wide":
+        masses = np.exp(rng.uniform(np.log(1e-5), np.log(10.0), n)).astype(np.float32)
+    got, ref = deposit_case(torch, d, box, pos, masses, 1.0, prec)
+    bad = ~np.isfinite(ref)
+    assert bad.sum() >= 8
+    assert np.array_equal(~np.isfinite(got), bad)
+    tol = 1e-9 if prec == "f64" else 2e-7
+    assert np.abs(got[~bad] - ref[~bad]).max() <= tol * np.abs(ref[~bad]).max()
+    p = genpk_amd.Plan(d, prec)
+    p.deposit(to_dev(torch, pos), box, None if masses is None else to_dev(torch, masses), 1.0)
+    P, c, _ = p.power(d, mass=float(n))
+    P, c = P.cpu().numpy(), c.cpu().numpy()
+    assert np.isnan(P[c > 0]).all()
+
+
 def test_deposit_accumulates_and_exports_into(gpu):
     import torch
     announce("deposit accumulate d=32")

@@ -220,6 +220,36 @@ def test_cic_periodic_wrap_and_accumulate():
     assert abs(a.sum() - 2.0) < 1e-12
 
 
+@pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf, 3e9])
+def test_cic_nonfinite_coordinate_cells(bad):
+    """fieldize.cpp:67 stores floor(x) in an int: on x86-64 a NaN, an infinity
+    or a floor beyond the int range converts to INT_MIN, and the cells are
+    then INT_MIN and INT_MIN + 1 taken modulo d with C's truncating %, made
+    non-negative (:70-75).  For d = 12 that is cells 4 and 5 (INT_MIN % 12 =
+    -8).  A non-finite coordinate makes all eight weights non-finite; a
+    huge finite one gives finite weights at those cells."""
+    d, box = 12, 12.0
+    imin = -2 ** 31
+
+    def cmod(a, m):  # C's % (truncation toward zero), then the reference's fix-up
+        r = a - m * int(a / m)
+        return r + m if r < 0 else r
+
+    assert (cmod(imin, d), cmod(imin + 1, d)) == (4, 5)
+    p = np.array([[bad, 2.5, 7.25]], np.float32)
+    f = oracle.new_field(d)
+    oracle.fieldize(box, d, f, p, None, 1.0)
+    touched = np.zeros_like(f, dtype=bool)
+    for x in (4, 5):
+        for y in (2, 3):
+            for z in (7, 8):
+                touched[x, y, z] = True
+    if np.isfinite(bad):
+        assert np.array_equal(f != 0, touched & (f != 0)) and (f[touched] != 0).any()
+    else:
+        assert np.array_equal(~np.isfinite(f), touched)
+
+
 # ---------------------------------------------------------------- binning closed forms
 
 def test_plane_wave_power_closed_form():
