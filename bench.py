@@ -265,7 +265,8 @@ def batched(genpk_amd, torch, prec, count, reps=5):
     n_side log-uniform in [64, 256] (n_side^3 particles), dims uniform over
     {128, 256, 384, 512}, 0.3-cell Gaussian lattice in a 100 Mpc/h box,
     nrbins = dims.  Plain: one plan per dims, each realisation run alone
-    (clear + deposit + fused power, then synchronize), times summed.
+    (clear + deposit + fused power, then synchronize), times summed; with
+    the default plan options and with the options the autotuner picked.
     Batched: the autotuned hipGraph over all realisations, one launch."""
     from genpk_amd import batch as gb
     from genpk_amd import synth
@@ -275,27 +276,35 @@ def batched(genpk_amd, torch, prec, count, reps=5):
         pos = synth.gaussian_lattice_torch(ns, box, d, 0.3, seed=sd)
         jobs.append(dict(pos=pos, boxsize=box, dims=d, nrbins=d, total_mass=float(pos.shape[0]), precision=prec))
     n_tot = sum(j["pos"].shape[0] for j in jobs)
-    plans = {d: genpk_amd.Plan(d, prec) for d in sorted({j["dims"] for j in jobs})}
 
-    def plain():
-        outs = []
-        for j in jobs:
-            p = plans[j["dims"]]
-            p.clear()
-            p.deposit(j["pos"], box)
-            outs.append(p.power(j["nrbins"], mass=j["total_mass"])[0])
-            torch.cuda.synchronize()
-        return outs
+    def plain_time(options):
+        plans = {d: genpk_amd.Plan(d, prec) for d in sorted({j["dims"] for j in jobs})}
+        if options:
+            for d, p in plans.items():
+                o = options[f"{d}/{prec}"]
+                p.set_options(o["tiles"], o["fft_variant"])
 
-    plain()
-    t_plain = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        ref = plain()
-        t_plain.append(time.perf_counter() - t0)
-    t_plain = sorted(t_plain)[1]
-    del plans
-    torch.cuda.empty_cache()
+        def plain():
+            outs = []
+            for j in jobs:
+                p = plans[j["dims"]]
+                p.clear()
+                p.deposit(j["pos"], box)
+                outs.append(p.power(j["nrbins"], mass=j["total_mass"])[0])
+                torch.cuda.synchronize()
+            return outs
+
+        plain()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            outs = plain()
+            ts.append(time.perf_counter() - t0)
+        del plans
+        torch.cuda.empty_cache()
+        return sorted(ts)[1], outs
+
+    t_plain, _ = plain_time(None)
     best, rep = gb.tune(jobs, use_cache=False, log=log)
     ts = []
     for _ in range(reps):
@@ -304,14 +313,17 @@ def batched(genpk_amd, torch, prec, count, reps=5):
         torch.cuda.synchronize()
         ts.append(time.perf_counter() - t0)
     t_b = sorted(ts)[reps // 2]
+    t_plain_tuned, ref = plain_time(rep["per_dims"])
     dev = max(float(((best.results(i)[0] - ref[i]).abs() / ref[i].abs().clamp(min=1e-300)).max())
               for i in range(len(jobs)))
     best.close()
     torch.cuda.empty_cache()
     return {"realisations": count, "particles": n_tot, "dims": sorted({j["dims"] for j in jobs}),
-            "ms_plain_sum": round(t_plain * 1e3, 3), "ms_batched": round(t_b * 1e3, 3),
-            "speedup": round(t_plain / t_b, 3), "Gparticles_per_s_batched": round(n_tot / t_b / 1e9, 3),
-            "autotune": rep, "max_rel_dev_from_plain": dev}
+            "ms_plain_sum": round(t_plain * 1e3, 3), "ms_plain_sum_tuned_options": round(t_plain_tuned * 1e3, 3),
+            "ms_batched": round(t_b * 1e3, 3), "speedup": round(t_plain / t_b, 3),
+            "speedup_vs_plain_tuned_options": round(t_plain_tuned / t_b, 3),
+            "Gparticles_per_s_batched": round(n_tot / t_b / 1e9, 3),
+            "autotune": rep, "max_rel_dev_from_plain_same_options": dev}
 
 
 def cross_512(genpk_amd, torch, prec, steps=10, warmup=2):

@@ -22,7 +22,7 @@ FLOAT32 = 4
 FLOAT64 = 8
 
 __all__ = ["Plan", "GenPKError", "fieldize", "invwindow", "powerspectrum", "fft_r2c_3d_inplace", "nexttwo",
-           "type_str", "print_pk", "FLOAT32", "FLOAT64", "lib"]
+           "type_str", "print_pk", "radix_plan", "FLOAT32", "FLOAT64", "lib"]
 
 
 def _precision(p):
@@ -64,6 +64,24 @@ class Plan:
             self.grid = torch.empty((self.dims, self.dims, self.rowlen), dtype=rdt, device=self.device)
             assert self.grid.numel() * self.grid.element_size() == nbytes.value
             check(lib().genpk_plan_attach_field(self._h, ctypes.c_void_p(self.grid.data_ptr())))
+
+    # -- tuning options -------------------------------------------------
+    def set_options(self, tiles=(16, 32, 32), fft_variant=0):
+        """Deposit tile edges (powers of two, clamped to dims) and the FFT
+        radix ordering (0: radix 8 first, 1: leftover radices first, 2: radix
+        4 first).  Drops the plan's deposits."""
+        self._sync_stream()
+        self._kept = None
+        self.generation += 1
+        tx, ty, tz = (int(t) for t in tiles)
+        check(lib().genpk_plan_set_options(self._h, tx, ty, tz, int(fft_variant)))
+        return self
+
+    def options(self):
+        """{"tiles": (tx, ty, tz) in effect, "fft_variant": v}."""
+        v = [ctypes.c_int() for _ in range(4)]
+        check(lib().genpk_plan_get_options(self._h, *[ctypes.byref(x) for x in v]))
+        return {"tiles": (v[0].value, v[1].value, v[2].value), "fft_variant": v[3].value}
 
     # -- plumbing -------------------------------------------------------
     def _sync_stream(self):
@@ -310,6 +328,15 @@ def fft_r2c_3d_inplace(field):
     assert field.dtype == np.float64 and field.flags.c_contiguous and field.shape == (d, d, 2 * (d // 2 + 1))
     check(lib().genpk_fft_r2c_3d_inplace(d, _np_dp(field)))
     return field.view(np.complex128)
+
+
+def radix_plan(n, fft_variant=0):
+    """Radices, in stage order, of the library's length-n transform under an
+    FFT variant (host only)."""
+    buf = (ctypes.c_int * 32)()
+    cnt = ctypes.c_int()
+    check(lib().genpk_radix_plan(int(n), int(fft_variant), buf, 32, ctypes.byref(cnt)))
+    return [buf[i] for i in range(cnt.value)]
 
 
 def nexttwo(n):

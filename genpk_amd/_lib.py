@@ -25,6 +25,9 @@ SIGNATURES = {
     "genpk_plan_field_bytes": (_c_int, [_c_vp, ctypes.POINTER(_c_sz)]),
     "genpk_plan_field_ptr": (_c_int, [_c_vp, ctypes.POINTER(_c_vp)]),
     "genpk_plan_attach_field": (_c_int, [_c_vp, _c_vp]),
+    "genpk_plan_set_options": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_int]),
+    "genpk_plan_get_options": (_c_int, [_c_vp, _ip, _ip, _ip, _ip]),
+    "genpk_radix_plan": (_c_int, [_c_i64, _c_int, _ip, _c_int, _ip]),
     "genpk_clear": (_c_int, [_c_vp]),
     "genpk_deposit": (_c_int, [_c_vp, _c_dbl, _c_i64, _c_vp, _c_vp, _c_dbl]),
     "genpk_deposit_retain": (_c_int, [_c_vp, _c_dbl, _c_i64, _c_vp, _c_vp, _c_dbl]),
@@ -42,6 +45,7 @@ SIGNATURES = {
     "genpk_batch_create": (_c_int, [ctypes.POINTER(_c_vp), _c_int, _c_vp]),
     "genpk_batch_add": (_c_int, [_c_vp, _c_i64, _c_int, _c_dbl, _c_i64, _c_vp, _c_vp, _c_dbl, _c_int, _c_dbl, _c_vp,
                                  _c_vp, _c_vp]),
+    "genpk_batch_plan_options": (_c_int, [_c_vp, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_int]),
     "genpk_batch_build": (_c_int, [_c_vp, _c_vp]),
     "genpk_batch_launch": (_c_int, [_c_vp]),
     "genpk_batch_branch_of": (_c_int, [_c_vp, _c_i64, _ip]),
@@ -55,6 +59,8 @@ SIGNATURES = {
     "invwindow": (_c_dbl, [_c_i64, _c_i64, _c_i64, _c_i64]),
     "powerspectrum": (_c_int, [_c_i64, _dp, _dp, _c_int, _dp, _ip, _dp, _c_dbl, _c_dbl]),
     "genpk_fft_r2c_3d_inplace": (_c_int, [_c_i64, _dp]),
+    "genpk_compat_resident": (_c_int, [_c_int]),
+    "genpk_compat_flush": (_c_int, []),
     # C++ linkage, gen-pk.h:93
     "_Z8fieldizediPdlPfS0_di": (_c_int, [_c_dbl, _c_int, _dp, _c_i64, _fp, _fp, _c_dbl, _c_int]),
 }

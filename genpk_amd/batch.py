@@ -3,11 +3,15 @@
 ``Batch`` wraps the genpk_batch_* entry points: each ``add`` is one
 forward (clear + deposit + fused r2c/bin) of its own dims^3 plan;
 ``build`` captures all of them into one graph whose branches run side by
-side; ``launch`` replays it with one call.  ``tune`` is the autotuner: it
-builds the batch under several configurations (number of parallel
-branches x job-to-branch policy), times real graph launches, keeps the
-fastest and remembers the choice in a JSON cache keyed by the batch's
-shape.  No counterpart in the reference (one realisation per process).
+side; ``launch`` replays it with one call.  ``tune`` is the autotuner, in
+two stages: per grid size and precision it times a plain forward of the
+largest job of that size under every deposit tile shape and FFT radix
+ordering (``tune_plan_options``); with those options it then builds the
+batch under several configurations (number of parallel branches x
+job-to-branch policy) and times real graph launches.  It keeps the fastest
+and remembers both choices in a JSON cache (per grid size, precision and
+device; per batch shape and device).  No counterpart in the reference (one
+realisation per process).
 """
 import ctypes
 import hashlib
@@ -19,7 +23,7 @@ import time
 from ._lib import check, lib
 from . import _precision, _ptr
 
-__all__ = ["Batch", "tune", "cache_path"]
+__all__ = ["Batch", "tune", "tune_plan_options", "cache_path", "TILE_CANDIDATES", "FFT_VARIANTS"]
 
 
 class Batch:
@@ -54,6 +58,13 @@ class Batch:
                                     _ptr(out[1]), _ptr(out[2])))
         self.jobs.append((dict(dims=int(dims), n=n, precision=precision, nrbins=nrbins), out, (pos, masses)))
         return len(self.jobs) - 1
+
+    def plan_options(self, dims, precision, tiles, fft_variant):
+        """Deposit tile edges and FFT radix variant of the batch's plans of one
+        grid size and precision (Plan.set_options), before build()."""
+        tx, ty, tz = (int(t) for t in tiles)
+        check(lib().genpk_batch_plan_options(self._h, int(dims), _precision(precision), tx, ty, tz,
+                                             int(fft_variant)))
 
     def build(self, assignment=None):
         """Capture the graph.  assignment: branch of each job (list of ints) or
@@ -162,19 +173,99 @@ def _time_batch(b, torch, reps):
     return ts[len(ts) // 2]
 
 
-def tune(add_args, candidates=(1, 2, 4, 8), policies=("lpt", "grouped"), reps=3, use_cache=True, log=None):
+# deposit tile shapes tried per grid size (x, y, z edges; clamped to dims)
+TILE_CANDIDATES = ((16, 32, 32), (8, 32, 32), (16, 16, 32), (8, 16, 32))
+FFT_VARIANTS = (0, 1, 2)
+
+
+def _plan_key(dims, precision, torch):
+    dev = torch.cuda.get_device_name(torch.cuda.current_device())
+    return "plan:" + json.dumps([dev, int(dims), "f32" if _precision(precision) == 4 else "f64"])
+
+
+def tune_plan_options(add_args, tiles=TILE_CANDIDATES, fft_variants=FFT_VARIANTS, reps=3, use_cache=True, log=None):
+    """Stage 1 of the autotuner: for every (dims, precision) of the jobs,
+    time a plain forward (clear + deposit + fused r2c/bin, synchronised) of
+    the largest job of that size under each deposit tile shape and FFT radix
+    variant (median of `reps` after one warm-up), and keep the fastest.
+    Variants that give the same radices as an earlier one are skipped.
+    Returns {(dims, precision): {"tiles", "fft_variant", "radices", ...}}."""
+    import torch
+    from . import Plan, radix_plan
+    cache = _load_cache(cache_path()) if use_cache else {}
+    groups = {}
+    for a in add_args:
+        key = (int(a["dims"]), _precision(a.get("precision", "f32")))
+        if key not in groups or a["pos"].shape[0] > groups[key]["pos"].shape[0]:
+            groups[key] = a
+    out = {}
+    for (d, prec), a in sorted(groups.items()):
+        ck = _plan_key(d, prec, torch)
+        if use_cache and ck in cache:
+            out[(d, prec)] = dict(cache[ck], cached=True)
+            continue
+        n = a["pos"].shape[0]
+        nrbins = int(a.get("nrbins") or d)
+        mass = float(a.get("mass", 1.0))
+        tm = a.get("total_mass") or mass * n
+        seen, timings, best = set(), {}, None
+        plan = Plan(d, prec)
+        for v in fft_variants:
+            rad = (tuple(radix_plan(d, v)), tuple(radix_plan(d // 2, v)))
+            if rad in seen:
+                continue
+            seen.add(rad)
+            for t in tiles:
+                plan.set_options(t, v)
+                eff = plan.options()["tiles"]
+
+                def fwd():
+                    plan.clear()
+                    plan.deposit(a["pos"], a["boxsize"], a.get("masses"), mass)
+                    plan.power(nrbins, mass=tm)
+                    torch.cuda.synchronize()
+
+                fwd()
+                ts = []
+                for _ in range(reps):
+                    t0 = time.perf_counter()
+                    fwd()
+                    ts.append(time.perf_counter() - t0)
+                ms = sorted(ts)[reps // 2] * 1e3
+                name = f"{eff[0]}x{eff[1]}x{eff[2]}/v{v}"
+                timings[name] = round(ms, 4)
+                if log:
+                    log(f"autotune d={d} {'f32' if prec == 4 else 'f64'}: tiles {name}: {ms:.3f} ms")
+                if best is None or ms < best[0]:
+                    best = (ms, list(eff), v, [list(rad[0]), list(rad[1])])
+        del plan
+        torch.cuda.empty_cache()
+        choice = {"tiles": best[1], "fft_variant": best[2], "radices_d": best[3][0], "radices_half": best[3][1],
+                  "ms": round(best[0], 4), "timings_ms": timings, "representative_n": int(n)}
+        out[(d, prec)] = dict(choice, cached=False)
+        if use_cache:
+            cache[ck] = choice
+    if use_cache:
+        _store_cache(cache_path(), cache)
+    return out
+
+
+def tune(add_args, candidates=(1, 2, 4, 8), policies=("lpt", "grouped"), reps=3, use_cache=True, log=None,
+         tiles=TILE_CANDIDATES, fft_variants=FFT_VARIANTS):
     """Build the fastest batch for the given jobs.
 
-    add_args: list of dicts of Batch.add keyword arguments.  Every
-    (branches, policy) candidate is built and timed over `reps` graph
-    launches (median); the winner is returned built, with a report
-    {"choice": ..., "timings_ms": ..., "cached": bool}.  The choice is cached
-    under the batch's shape (grid sizes, precisions, bins, particle-count
-    octaves, device) so later batches of that shape build it directly.
+    add_args: list of dicts of Batch.add keyword arguments.  Stage 1 picks
+    each grid size's deposit tile shape and FFT radix variant
+    (tune_plan_options); stage 2 builds every (branches, policy) candidate
+    with those options and times it over `reps` graph launches (median).
+    The winner is returned built, with a report {"per_dims": ..., "choice":
+    ..., "timings_ms": ..., "cached": bool}.  Both stages cache their choices
+    (per grid size, precision and device; per batch shape and device).
     """
     import torch
     meta = [dict(dims=int(a["dims"]), n=int(a["pos"].shape[0]), precision=a.get("precision", "f32"),
                  nrbins=int(a.get("nrbins") or a["dims"])) for a in add_args]
+    per_dims = tune_plan_options(add_args, tiles, fft_variants, reps, use_cache, log)
     sig = _signature(meta, torch)
     path = cache_path()
     cache = _load_cache(path) if use_cache else {}
@@ -183,12 +274,16 @@ def tune(add_args, candidates=(1, 2, 4, 8), policies=("lpt", "grouped"), reps=3,
         b = Batch(branches)
         for a in add_args:
             b.add(**a)
+        for (d, prec), ch in per_dims.items():
+            b.plan_options(d, prec, ch["tiles"], ch["fft_variant"])
         b.build(None if policy == "lpt" else _grouped_assignment(meta, branches))
         return b
 
+    report_dims = {f"{d}/{'f32' if p == 4 else 'f64'}": ch for (d, p), ch in per_dims.items()}
     if use_cache and sig in cache:
         ch = cache[sig]
-        return make(ch["branches"], ch["policy"]), {"choice": ch, "timings_ms": None, "cached": True}
+        return make(ch["branches"], ch["policy"]), {"per_dims": report_dims, "choice": ch, "timings_ms": None,
+                                                    "cached": True}
 
     best, best_t, timings = None, None, {}
     for br in candidates:
@@ -208,6 +303,7 @@ def tune(add_args, candidates=(1, 2, 4, 8), policies=("lpt", "grouped"), reps=3,
                 b.close()
             torch.cuda.empty_cache()
     if use_cache:
+        cache = _load_cache(path)
         cache[sig] = choice
         _store_cache(path, cache)
-    return best, {"choice": choice, "timings_ms": timings, "cached": False}
+    return best, {"per_dims": report_dims, "choice": choice, "timings_ms": timings, "cached": False}

@@ -63,12 +63,6 @@ def build(verbose=False):
                         "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(rocm, "lib"), *hdf5["libs"],
                         "-Wl,-rpath,$ORIGIN/_hdf5lib"], check=True)
         os.replace(CLI + ".tmp", CLI)
-    if hdf5["cflags"]:
-        helper = os.path.join(HERE, "..", "tests", "helpers", "hdf5_snap")
-        subprocess.run(["g++", "-O1", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "cli"), *hdf5["cflags"],
-                        helper + ".cpp", "-o", helper + ".tmp", *hdf5["libs"],
-                        "-Wl,-rpath,$ORIGIN/../../genpk_amd/_hdf5lib"], check=True)
-        os.replace(helper + ".tmp", helper)
     return LIB
 
 

@@ -382,7 +382,7 @@ void adjoint_x_impl(Plan &p, const AdjWeight &w) {
   using C = typename Cx<T>::C;
   const int B = col_block(p.prec, p.d, p.hz);
   const int nkb = (int)ceil_div(p.hz, B);
-  const RadixPlan rp = make_radix_plan((int)p.d);
+  const RadixPlan &rp = p.rp_d;
   const C *tw = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
   GENPK_REQUIRE((int64_t)p.d * B <= (int64_t)AdjCfg<T>::MAXE * ADJ_THREADS, "adjoint x pass: column block too large");
   Stage st(p, "adjoint_x");

@@ -16,6 +16,7 @@
 #include "../../include/genpk_amd.h"
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <memory>
 #include <tuple>
@@ -47,6 +48,8 @@ struct Batch {
   std::vector<hipStream_t> bstreams;
   std::vector<hipEvent_t> events;
   std::map<std::tuple<int, int64_t, int>, std::unique_ptr<Plan>> plans;
+  // per (dims, precision): deposit tile edges x, y, z and FFT radix variant
+  std::map<std::pair<int64_t, int>, std::array<int, 4>> options;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 
@@ -69,6 +72,11 @@ struct Batch {
     auto it = plans.find(key);
     if (it != plans.end()) return *it->second;
     auto p = std::make_unique<Plan>(j.d, j.prec, bstreams[j.branch]);
+    auto o = options.find(std::make_pair(j.d, j.prec));
+    if (o != options.end()) {
+      plan_set_tiles(*p, o->second[0], o->second[1], o->second[2]);
+      plan_set_fft_variant(*p, o->second[3]);
+    }
     Plan &r = *p;
     plans[key] = std::move(p);
     return r;
@@ -201,6 +209,23 @@ int genpk_batch_add(genpk_batch *batch, int64_t dims, int precision, double boxs
     GENPK_REQUIRE(power != nullptr, "null power output");
     BatchJob j{dims, precision, boxsize, n, pos, masses, mass, nrbins, total_mass, power, count, keffs};
     batch->b.jobs.push_back(j);
+  });
+}
+
+int genpk_batch_plan_options(genpk_batch *batch, int64_t dims, int precision, int tile_x, int tile_y, int tile_z,
+                             int fft_variant) {
+  return guarded([&] {
+    GENPK_REQUIRE(batch != nullptr, "null batch");
+    GENPK_REQUIRE(batch->b.exec == nullptr, "batch already built");
+    GENPK_REQUIRE(precision == GENPK_FLOAT32 || precision == GENPK_FLOAT64, "bad precision");
+    // the rules of genpk_plan_set_options
+    auto pow2 = [](int v) { return v >= 1 && (v & (v - 1)) == 0; };
+    GENPK_REQUIRE(pow2(tile_x) && pow2(tile_y) && pow2(tile_z) && tile_z >= 2,
+                  "tile edges must be powers of two (z edge >= 2)");
+    GENPK_REQUIRE((int64_t)(tile_x + 1) * (tile_y + 1) * (tile_z + 1) * 8 <= 160 * 1024,
+                  "a halo'd tile of 64-bit accumulators must fit in LDS (160 KB)");
+    GENPK_REQUIRE(fft_variant >= 0 && fft_variant < FFT_VARIANTS, "unknown FFT radix variant");
+    batch->b.options[std::make_pair(dims, precision)] = {tile_x, tile_y, tile_z, fft_variant};
   });
 }
 

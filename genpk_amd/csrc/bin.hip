@@ -554,7 +554,7 @@ static size_t fused_lds(const Plan &a, int B, int nrbins, bool pp) {
 // Whether the REG1 instance applies (one field with ping-pong buffers).
 template <typename T>
 static bool fused_reg1(const Plan &a, int B, bool pp) {
-  const RadixPlan rp = make_radix_plan((int)a.d);
+  const RadixPlan &rp = a.rp_d;
   return pp && rp.st[0].R == BinCfg<T>::MAXE && a.d * B == (int64_t)BIN_THREADS * BinCfg<T>::MAXE && rp.nst % 2 == 0;
 }
 
@@ -577,7 +577,7 @@ static void launch_fused(Plan &a, Plan *b, const BinTables &bt, int B, bool keep
   // the R2L instance finishes the transform only in registers for the binning
   GENPK_REQUIRE(!(R2L && keep), "the fused pass cannot keep the spectrum with the last stage in the binning");
   const int nkb = (int)ceil_div(a.hz, B);
-  const RadixPlan rp = make_radix_plan((int)a.d);
+  const RadixPlan &rp = a.rp_d;
   const C *tw = (a.prec == 4) ? (const C *)a.fft.tw_d_f32.ptr : (const C *)a.fft.tw_d_f64.ptr;
   BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), bt.thr.as<int64_t>(), a.wtab.as<float>(), bt.lq};
   const size_t lds = fused_lds<T, NF>(a, B, bt.nrbins, pp);
@@ -618,7 +618,7 @@ static FusedShape fused_shape(const Plan &a, const Plan *b, int nrbins) {
                             : fused_lds<double, 1>(a, B, nrbins, true)) <= (size_t)LDS_LIMIT;
   s.reg1 = !b && (a.prec == 4 ? fused_reg1<float>(a, B, s.pp) : fused_reg1<double>(a, B, s.pp));
   // FP32 with a last stage of radix 2: that stage runs in the binning
-  const RadixPlan xrp = make_radix_plan((int)a.d);
+  const RadixPlan &xrp = a.rp_d;
   s.r2l = s.reg1 && a.prec == 4 && xrp.st[xrp.nst - 1].R == 2;
   const bool pp = s.pp;
   if (a.prec == 4)

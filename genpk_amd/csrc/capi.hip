@@ -7,6 +7,7 @@
 #include "../../include/genpk_compat.h"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -33,20 +34,33 @@ Plan::Plan(int64_t dims, int precision, hipStream_t s) {
   hz = d / 2 + 1;
   rowlen = 2 * hz;
   field_elems = (size_t)d * d * rowlen;
-  TileGeom &t = tg;
-  // tile edges: powers of two, at most 16 x 32 x 32 and at most d
-  auto pow2_le = [](int64_t cap, int64_t v, int &lg) {
-    int64_t t = 1;
+  plan_set_tiles(*this, 16, 32, 32);
+  plan_set_fft_variant(*this, 0);
+  field.ensure(field_elems * prec);
+  const bool fftable = (d % 2 == 0) && fft_length_ok(d) && fft_length_ok(d / 2) && d >= 2;
+  if (fftable) fft_tables_init(*this);
+}
+
+void plan_set_tiles(Plan &p, int tx, int ty, int tz) {
+  auto pow2 = [](int v) { return v >= 1 && (v & (v - 1)) == 0; };
+  GENPK_REQUIRE(pow2(tx) && pow2(ty) && pow2(tz) && tz >= 2, "tile edges must be powers of two (z edge >= 2)");
+  GENPK_REQUIRE((int64_t)(tx + 1) * (ty + 1) * (tz + 1) * 8 <= 160 * 1024,
+                "a halo'd tile of 64-bit accumulators must fit in LDS (160 KB)");
+  const int64_t d = p.d;
+  TileGeom t;
+  // each edge: the requested power of two, at most the largest power of two <= d
+  auto clamp = [d](int cap, int &lg) {
+    int64_t e = 1;
     lg = 0;
-    while (t * 2 <= std::min(cap, v)) {
-      t *= 2;
+    while (e * 2 <= std::min<int64_t>(cap, d)) {
+      e *= 2;
       lg++;
     }
-    return (int)t;
+    return (int)e;
   };
-  t.TX = pow2_le(16, d, t.sx);
-  t.TY = pow2_le(32, d, t.sy);
-  t.TZ = pow2_le(32, d, t.sz);
+  t.TX = clamp(tx, t.sx);
+  t.TY = clamp(ty, t.sy);
+  t.TZ = clamp(tz, t.sz);
   t.ntx = (int)ceil_div(d, t.TX);
   t.nty = (int)ceil_div(d, t.TY);
   t.ntz = (int)ceil_div(d, t.TZ);
@@ -67,10 +81,19 @@ Plan::Plan(int64_t dims, int precision, hipStream_t s) {
     t.lo_bits = (t.kbits + 1) / 2;
     t.hi_bits = t.kbits - t.lo_bits;
   }
-  GENPK_REQUIRE(t.hi_bits <= 11 && t.lo_bits <= 11, "tile key too wide");
-  field.ensure(field_elems * prec);
-  const bool fftable = (d % 2 == 0) && fft_length_ok(d) && fft_length_ok(d / 2) && d >= 2;
-  if (fftable) fft_tables_init(*this);
+  GENPK_REQUIRE(t.hi_bits <= 11 && t.lo_bits <= 11, "tile key too wide: use larger tiles");
+  if (p.tiles.ptr) GENPK_HIP(hipStreamSynchronize(p.stream));  // queued kernels may still use the store
+  p.tg = t;
+  p.tiles.release();  // sized for the old tiling
+  p.tiles_live = false;
+  p.order.valid = false;
+}
+
+void plan_set_fft_variant(Plan &p, int variant) {
+  GENPK_REQUIRE(variant >= 0 && variant < FFT_VARIANTS, "unknown FFT radix variant");
+  p.fft_variant = variant;
+  p.rp_d = make_radix_plan((int)p.d, variant);
+  p.rp_h = make_radix_plan((int)std::max<int64_t>(p.d / 2, 1), variant);
 }
 
 bool plan_fft_ok(const Plan &p) {
@@ -79,15 +102,50 @@ bool plan_fft_ok(const Plan &p) {
 
 
 // ---- cached host-surface plans for the compatibility shims ----
+// Resident mode (off unless GENPK_COMPAT_RESIDENT=1 or genpk_compat_resident
+// turns it on): the grid of the shims stays in the compat plan between
+// calls on the same host array.  State:
+//   Density: `host`'s contents were loaded into the plan's grid and later
+//            fieldize calls added deposits there (the host array lacks them);
+//   Spectrum: the plan's grid holds the spectrum the last in-place FFT of
+//            `host` wrote back to it (both copies equal).
+enum class Resident { None, Density, Spectrum };
+
 struct CompatCache {
   std::mutex mu;
   std::map<int64_t, std::unique_ptr<Plan>> a, b;
   DevBuf pos, masses, grid;
   DevBuf pw, ke, ct;
+  bool resident_mode = false;
+  Resident res = Resident::None;
+  double *res_host = nullptr;
+  int64_t res_dims = 0;
 };
 static CompatCache &compat() {
   static CompatCache c;
+  static bool init = [] {
+    const char *e = std::getenv("GENPK_COMPAT_RESIDENT");
+    c.resident_mode = e && e[0] == '1';
+    return true;
+  }();
+  (void)init;
   return c;
+}
+
+static Plan &compat_plan(std::map<int64_t, std::unique_ptr<Plan>> &m, int64_t dims);
+
+// Bring the host array of a resident density up to date (deposits merged
+// into the grid, then copied back) and leave resident mode's state empty.
+static void flush_resident(CompatCache &cc) {
+  if (cc.res == Resident::Density) {
+    Plan &pa = compat_plan(cc.a, cc.res_dims);
+    tiles_to_field(pa);
+    GENPK_HIP(hipStreamSynchronize(pa.stream));
+    GENPK_HIP(hipMemcpy(cc.res_host, pa.field.ptr, pa.field_elems * 8, hipMemcpyDeviceToHost));
+  }
+  cc.res = Resident::None;
+  cc.res_host = nullptr;
+  cc.res_dims = 0;
 }
 static Plan &compat_plan(std::map<int64_t, std::unique_ptr<Plan>> &m, int64_t dims) {
   auto it = m.find(dims);
@@ -151,6 +209,39 @@ int genpk_plan_attach_field(genpk_plan *plan, void *ptr) {
     GENPK_HIP(hipStreamSynchronize(p.stream));
     p.field.attach(ptr, p.field_elems * p.prec);
     p.fstate = FieldState::None;
+  });
+}
+
+int genpk_plan_set_options(genpk_plan *plan, int tile_x, int tile_y, int tile_z, int fft_variant) {
+  return guarded([&] {
+    GENPK_REQUIRE(plan && plan->p, "null plan");
+    Plan &p = *plan->p;
+    GENPK_REQUIRE(fft_variant >= 0 && fft_variant < FFT_VARIANTS, "unknown FFT radix variant");
+    plan_set_tiles(p, tile_x, tile_y, tile_z);
+    plan_set_fft_variant(p, fft_variant);
+  });
+}
+
+int genpk_radix_plan(int64_t n, int fft_variant, int *radices, int max, int *count) {
+  return guarded([&] {
+    GENPK_REQUIRE(radices && count && max >= 0, "null argument");
+    GENPK_REQUIRE(fft_variant >= 0 && fft_variant < FFT_VARIANTS, "unknown FFT radix variant");
+    GENPK_REQUIRE(fft_length_ok(n), "transform length must be 2^a 3^b 5^c, at most 2^20");
+    const RadixPlan rp = make_radix_plan((int)n, fft_variant);
+    GENPK_REQUIRE(rp.nst <= max, "radices buffer too short");
+    for (int s = 0; s < rp.nst; s++) radices[s] = rp.st[s].R;
+    *count = rp.nst;
+  });
+}
+
+int genpk_plan_get_options(const genpk_plan *plan, int *tile_x, int *tile_y, int *tile_z, int *fft_variant) {
+  return guarded([&] {
+    GENPK_REQUIRE(plan && plan->p && tile_x && tile_y && tile_z && fft_variant, "null argument");
+    const Plan &p = *plan->p;
+    *tile_x = p.tg.TX;
+    *tile_y = p.tg.TY;
+    *tile_z = p.tg.TZ;
+    *fft_variant = p.fft_variant;
   });
 }
 
@@ -401,7 +492,12 @@ int powerspectrum(int64_t dims, double (*outfield)[2], double (*outfield2)[2], i
     GENPK_REQUIRE(outfield && outfield2 && power && count && keffs, "null argument");
     Plan &pa = compat_plan(cc.a, dims);
     const size_t bytes = pa.field_elems * 8;
-    GENPK_HIP(hipMemcpy(pa.field.ptr, outfield, bytes, hipMemcpyHostToDevice));
+    const bool kept = cc.res == Resident::Spectrum && cc.res_host == (double *)outfield && cc.res_dims == dims &&
+                      outfield2 == outfield;
+    if (!kept) {
+      flush_resident(cc);
+      GENPK_HIP(hipMemcpy(pa.field.ptr, outfield, bytes, hipMemcpyHostToDevice));
+    }
     pa.fstate = FieldState::Spectrum;
     pa.tiles_live = false;
     double *dp = static_cast<double *>(cc.pw.ensure((size_t)nrbins * 8 * 3));
@@ -434,12 +530,35 @@ int genpk_fft_r2c_3d_inplace(int64_t dims, double *field) {
     Plan &pa = compat_plan(cc.a, dims);
     GENPK_REQUIRE(plan_fft_ok(pa), "the FFT needs an even dims with dims/2 = 2^a 3^b 5^c");
     const size_t bytes = pa.field_elems * 8;
-    GENPK_HIP(hipMemcpy(pa.field.ptr, field, bytes, hipMemcpyHostToDevice));
-    pa.fstate = FieldState::Real;
-    pa.tiles_live = false;
+    if (!(cc.res == Resident::Density && cc.res_host == field && cc.res_dims == dims)) {
+      flush_resident(cc);
+      GENPK_HIP(hipMemcpy(pa.field.ptr, field, bytes, hipMemcpyHostToDevice));
+      pa.fstate = FieldState::Real;
+      pa.tiles_live = false;
+    }
+    // resident density: the loaded grid plus the deposits kept in the tiles
     r2c(pa);
+    GENPK_HIP(hipStreamSynchronize(pa.stream));
     GENPK_HIP(hipMemcpy(field, pa.field.ptr, bytes, hipMemcpyDeviceToHost));
+    cc.res = cc.resident_mode ? Resident::Spectrum : Resident::None;
+    cc.res_host = cc.resident_mode ? field : nullptr;
+    cc.res_dims = cc.resident_mode ? dims : 0;
   });
+}
+
+int genpk_compat_resident(int on) {
+  CompatCache &cc = compat();
+  std::lock_guard<std::mutex> lk(cc.mu);
+  return guarded([&] {
+    if (!on) flush_resident(cc);
+    cc.resident_mode = on != 0;
+  });
+}
+
+int genpk_compat_flush(void) {
+  CompatCache &cc = compat();
+  std::lock_guard<std::mutex> lk(cc.mu);
+  return guarded([&] { flush_resident(cc); });
 }
 
 }  // extern "C"
@@ -458,6 +577,8 @@ int fieldize(double boxsize, int dims, double *out, int64_t segment_particles, f
     const int64_t n = segment_particles;
     if (n <= 0) return;
     GENPK_REQUIRE(positions != nullptr, "null positions");
+    // the previous call's deposit may still read the staging buffers
+    GENPK_HIP(hipStreamSynchronize(p.stream));
     float *dpos = static_cast<float *>(cc.pos.ensure((size_t)n * 12));
     GENPK_HIP(hipMemcpy(dpos, positions, (size_t)n * 12, hipMemcpyHostToDevice));
     float *dm = nullptr;
@@ -465,6 +586,22 @@ int fieldize(double boxsize, int dims, double *out, int64_t segment_particles, f
       dm = static_cast<float *>(cc.masses.ensure((size_t)n * 4));
       GENPK_HIP(hipMemcpy(dm, masses, (size_t)n * 4, hipMemcpyHostToDevice));
     }
+    if (cc.resident_mode && extra == 1) {
+      // the grid stays in the plan: loaded once, then deposits accumulate in
+      // its tiles until the in-place FFT or a flush
+      if (!(cc.res == Resident::Density && cc.res_host == out && cc.res_dims == dims)) {
+        flush_resident(cc);
+        GENPK_HIP(hipMemcpy(p.field.ptr, out, p.field_elems * 8, hipMemcpyHostToDevice));
+        p.fstate = FieldState::Real;
+        p.tiles_live = false;
+        cc.res = Resident::Density;
+        cc.res_host = out;
+        cc.res_dims = dims;
+      }
+      deposit(p, boxsize, n, dpos, dm, mass);
+      return;
+    }
+    flush_resident(cc);
     const size_t gbytes = (size_t)dims * dims * rowlen * 8;
     void *dg = cc.grid.ensure(gbytes);
     GENPK_HIP(hipMemcpy(dg, out, gbytes, hipMemcpyHostToDevice));

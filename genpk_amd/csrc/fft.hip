@@ -468,7 +468,7 @@ static void r2c_z_impl(Plan &p, bool from_tiles) {
   const int64_t M = p.d / 2;
   const int L = rows_per_block<T>(M);
   const int64_t nrows = p.d * p.d;
-  const RadixPlan rp = make_radix_plan((int)M);
+  const RadixPlan &rp = p.rp_h;
   const C *twM = (p.prec == 4) ? (const C *)p.fft.tw_h_f32.ptr : (const C *)p.fft.tw_h_f64.ptr;
   const C *twd = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
   const FoldGeom fg = fold_geom(p);
@@ -501,7 +501,7 @@ static void c2r_z_impl(Plan &p) {
   const int64_t M = p.d / 2;
   const int L = rows_per_block<T>(M);
   const int64_t nrows = p.d * p.d;
-  const RadixPlan rp = make_radix_plan((int)M);
+  const RadixPlan &rp = p.rp_h;
   const C *twM = (p.prec == 4) ? (const C *)p.fft.tw_h_f32.ptr : (const C *)p.fft.tw_h_f64.ptr;
   const C *twd = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
   Stage st(p, "ifft_z");
@@ -516,7 +516,7 @@ static void cols_impl(Plan &p, int axis) {
   using C = typename Cx<T>::C;
   const int B = col_block(p.prec, p.d, p.hz);
   const int nkb = (int)ceil_div(p.hz, B);
-  const RadixPlan rp = make_radix_plan((int)p.d);
+  const RadixPlan &rp = p.rp_d;
   const C *tw = (p.prec == 4) ? (const C *)p.fft.tw_d_f32.ptr : (const C *)p.fft.tw_d_f64.ptr;
   const size_t lds = (size_t)p.d * (B + 1) * sizeof(C);
   const int64_t nblk = p.d * nkb;

@@ -147,11 +147,18 @@ struct RadixPlan {
   StagePlan st[14];
 };
 
-inline RadixPlan make_radix_plan(int n) {
+// Radix orderings (variant): 0 = radix 8 first, then 4, 2, 3, 5 (the
+// fewest stages; the leftover small radices last); 1 = the same radices
+// with the leftover powers of two, 3s and 5s first and the 8s last; 2 =
+// radix 4 first (no 8), then 2, 3, 5 (more stages, smaller butterflies).
+constexpr int FFT_VARIANTS = 3;
+
+inline RadixPlan make_radix_plan(int n, int variant = 0) {
   RadixPlan p;
   p.n = n;
   int r[32], nr = 0, m = n;
-  while (m % 8 == 0) { r[nr++] = 8; m /= 8; }
+  if (variant != 2)
+    while (m % 8 == 0) { r[nr++] = 8; m /= 8; }
   while (m % 4 == 0) { r[nr++] = 4; m /= 4; }
   while (m % 2 == 0) { r[nr++] = 2; m /= 2; }
   while (m % 3 == 0) { r[nr++] = 3; m /= 3; }
@@ -159,6 +166,15 @@ inline RadixPlan make_radix_plan(int n) {
   if (m != 1 || nr > 14) {
     p.nst = -1;  // unsupported prime factor
     return p;
+  }
+  if (variant == 1) {
+    // the 8s (leading) move behind the other radices
+    int e = 0;
+    while (e < nr && r[e] == 8) e++;
+    int t[32], k = 0;
+    for (int s = e; s < nr; s++) t[k++] = r[s];
+    for (int s = 0; s < e; s++) t[k++] = 8;
+    for (int s = 0; s < nr; s++) r[s] = t[s];
   }
   int ns = 1;
   for (int s = 0; s < nr; s++) {

@@ -2,6 +2,7 @@
 // cached tables for one grid size and precision.
 #pragma once
 #include "common.h"
+#include "fft_core.h"
 #include <map>
 #include <string>
 #include <memory>
@@ -149,6 +150,8 @@ struct Plan {
   int64_t rowlen = 0;       // 2*hz
   size_t field_elems = 0;   // d*d*rowlen reals
   TileGeom tg;
+  int fft_variant = 0;      // radix ordering of the transforms (make_radix_plan)
+  RadixPlan rp_d, rp_h;     // radix plans of length d (y, x passes) and d/2 (z passes)
 
   DevBuf field;             // padded grid, real or half-complex
   FieldState fstate = FieldState::None;
@@ -174,6 +177,13 @@ struct Plan {
 
   Plan(int64_t dims, int precision, hipStream_t s);
 };
+
+// Deposit tile edges of a plan (powers of two; each clamped to the largest
+// power of two <= d; TZ >= 2; the halo'd tile (TX+1)(TY+1)(TZ+1) of u64
+// must fit in LDS).  Drops the plan's deposits and kept tile order.
+void plan_set_tiles(Plan &p, int tx, int ty, int tz);
+// Radix ordering of the plan's transforms (make_radix_plan's variants).
+void plan_set_fft_variant(Plan &p, int variant);
 
 // RAII stage scope: records start/stop events when timing is enabled.
 struct Stage {

@@ -18,7 +18,10 @@ namespace {
 // ms for two 512-thread workgroups per CU staging 4096 records each).
 constexpr int SORT_THREADS = 1024;
 constexpr int SORT_WAVES = SORT_THREADS / 64;
-constexpr int SORT_CHUNK = 65536;                                   // records per workgroup
+// Records per workgroup: from SORT_CHUNK_MIN (one sub-chunk of the
+// downsweep) up to SORT_CHUNK_MAX, the smallest that keeps at most four
+// chunks per CU (small sets then still spread over every CU).
+constexpr int64_t SORT_CHUNK_MIN = 8192, SORT_CHUNK_MAX = 65536;
 
 // Source of records: either the caller's arrays (positions [n][3], optional
 // masses [n]) or a packed record buffer of RW floats per record.
@@ -62,10 +65,11 @@ struct ChunkMap {
   const uint32_t *chunk_start; // [nseg+1] first chunk of each segment
   int64_t nchunks;             // pass 1: number of chunks
   int nlo;                     // pass 2: digits
+  int64_t chunk;               // records per chunk (a multiple of the downsweep's sub-chunk)
   __device__ __forceinline__ bool range(int64_t g, int64_t &b, int64_t &e, int64_t &tb, int64_t &ts) const {
     if (nseg == 0) {
-      b = g * SORT_CHUNK;
-      e = min(n, b + SORT_CHUNK);
+      b = g * chunk;
+      e = min(n, b + chunk);
       tb = g;
       ts = nchunks;
       return b < e;
@@ -78,8 +82,8 @@ struct ChunkMap {
     }
     const int s = lo;
     const int64_t c0 = chunk_start[s];
-    b = (int64_t)seg_start[s] + (g - c0) * SORT_CHUNK;
-    e = min((int64_t)seg_start[s + 1], b + SORT_CHUNK);
+    b = (int64_t)seg_start[s] + (g - c0) * chunk;
+    e = min((int64_t)seg_start[s + 1], b + chunk);
     tb = c0 * nlo + (g - c0);
     ts = (int64_t)chunk_start[s + 1] - c0;
     return b < e;
@@ -445,7 +449,7 @@ __global__ void k_mrange_init(uint32_t *mrange) {
 }
 
 // Pass-2 segment bookkeeping from the scanned pass-1 table.
-__global__ void k_seginfo(const uint32_t *__restrict__ t1scan, int64_t g1, int nseg, int64_t n,
+__global__ void k_seginfo(const uint32_t *__restrict__ t1scan, int64_t g1, int nseg, int64_t n, int64_t chunk,
                           uint32_t *__restrict__ seg_start, uint32_t *__restrict__ nchunks) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s > nseg) return;
@@ -457,7 +461,7 @@ __global__ void k_seginfo(const uint32_t *__restrict__ t1scan, int64_t g1, int n
   const uint32_t st = t1scan[(int64_t)s * g1];
   const uint32_t en = (s + 1 < nseg) ? t1scan[(int64_t)(s + 1) * g1] : (uint32_t)n;
   seg_start[s] = st;
-  nchunks[s] = (en - st + SORT_CHUNK - 1) / SORT_CHUNK;
+  nchunks[s] = (uint32_t)(((int64_t)en - st + chunk - 1) / chunk);
 }
 
 // Two-pass tile starts from the scanned pass-2 table: the offset of a tile's
@@ -583,7 +587,9 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
 
   float *A = p.recA.as<float>();
   float *B = p.recB.as<float>();
-  const int64_t g1 = ceil_div(n, SORT_CHUNK);
+  int64_t chunk = SORT_CHUNK_MIN;
+  while (chunk < SORT_CHUNK_MAX && chunk * 4 * num_cus() < n) chunk *= 2;
+  const int64_t g1 = ceil_div(n, chunk);
   Src<NM> raw{pos, masses, true};
   uint32_t *perm1 = with_perm ? p.permA.as<uint32_t>() : nullptr;
   uint32_t *perm2 = (with_perm && tg.passes == 2) ? p.permB.as<uint32_t>() : nullptr;
@@ -603,7 +609,7 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     ks = nb;
     uint32_t *t1 = static_cast<uint32_t *>(p.table1.ensure((size_t)nb * g1 * 4));
     const DigitSel dsel = digit_sel(kg, 0, tg.hi_bits);
-    ChunkMap cm{n, 0, nullptr, nullptr, g1, 0};
+    ChunkMap cm{n, 0, nullptr, nullptr, g1, 0, chunk};
     {
       Stage st(p, "sort_upsweep1");
       hipLaunchKernelGGL((k_upsweep<NM>), dim3((unsigned)g1), dim3(SORT_THREADS), nb * 4, s, raw, cm, kg, dsel, nb, t1);
@@ -631,7 +637,7 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     // records
     uint8_t *dig2 = tg.lo_bits <= 8 && tg.hi_bits <= 8 ? static_cast<uint8_t *>(p.dig2.ensure((size_t)n + 64))
                                                         : nullptr;
-    ChunkMap cm1{n, 0, nullptr, nullptr, g1, 0};
+    ChunkMap cm1{n, 0, nullptr, nullptr, g1, 0, chunk};
     {
       Stage st(p, "sort_upsweep1");
       hipLaunchKernelGGL((k_upsweep<NM>), dim3((unsigned)g1), dim3(SORT_THREADS), nhi * 4, s, raw, cm1, kg, d1, nhi,
@@ -646,13 +652,13 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     // segments of pass 2
     uint32_t *seg = static_cast<uint32_t *>(p.seginfo.ensure((size_t)(nhi + 1) * 3 * 4));
     uint32_t *seg_start = seg, *nchunks = seg + (nhi + 1), *chunk_start = seg + 2 * (nhi + 1);
-    hipLaunchKernelGGL(k_seginfo, dim3((unsigned)ceil_div(nhi + 1, 256)), dim3(256), 0, s, t1, g1, nhi, n, seg_start,
-                       nchunks);
+    hipLaunchKernelGGL(k_seginfo, dim3((unsigned)ceil_div(nhi + 1, 256)), dim3(256), 0, s, t1, g1, nhi, n, chunk,
+                       seg_start, nchunks);
     GENPK_LAUNCH_CHECK();
     exclusive_scan_u32(nchunks, chunk_start, nhi + 1, p.scan_tmp, s);
     const int64_t g2 = g1 + nhi;  // upper bound of pass-2 chunks
     uint32_t *t2 = static_cast<uint32_t *>(p.table2.ensure((size_t)g2 * nlo * 4));
-    ChunkMap cm2{n, nhi, seg_start, chunk_start, 0, nlo};
+    ChunkMap cm2{n, nhi, seg_start, chunk_start, 0, nlo, chunk};
     Src<NM> pa{A, nullptr, false};
     {
       Stage st(p, "sort_upsweep2");

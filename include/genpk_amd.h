@@ -52,6 +52,23 @@ int genpk_plan_field_ptr(genpk_plan *plan, void **ptr);
  * own grid; the caller keeps it alive until the plan is destroyed. */
 int genpk_plan_attach_field(genpk_plan *plan, void *ptr);
 
+/* Tuning options of a plan (no counterpart in the reference; the batched
+ * form's autotuner picks them per grid size):
+ *   tile_x, tile_y, tile_z: deposit tile edges in cells, powers of two
+ *     (each clamped to the largest power of two <= dims; tile_z >= 2), whose
+ *     halo'd tile of 64-bit accumulators, (tile_x+1)(tile_y+1)(tile_z+1) x 8
+ *     bytes, fits in LDS (160 KB).  Default 16 x 32 x 32.
+ *   fft_variant: radix ordering of the transforms: 0 = radix 8 first (the
+ *     fewest stages, default), 1 = the same radices with the leftover 4, 2,
+ *     3, 5 first, 2 = radix 4 first (no radix 8).
+ * Changing the tiles drops the plan's deposits and its kept tile order. */
+int genpk_plan_set_options(genpk_plan *plan, int tile_x, int tile_y, int tile_z, int fft_variant);
+/* The plan's options in effect (tile edges after clamping to dims). */
+int genpk_plan_get_options(const genpk_plan *plan, int *tile_x, int *tile_y, int *tile_z, int *fft_variant);
+/* The radices, in stage order, of a length-n transform under fft_variant
+ * (host only; the y and x passes use n = dims, the z passes n = dims/2). */
+int genpk_radix_plan(int64_t n, int fft_variant, int *radices, int max, int *count);
+
 /* Empty the density (the reference zeroes the grid, gen-pk.cpp:208). */
 int genpk_clear(genpk_plan *plan);
 
@@ -159,12 +176,16 @@ int genpk_adjoint_gather(genpk_plan *plan, double boxsize, int64_t n, const floa
  *   genpk_batch_launch: one graph launch on the batch stream; the outputs are
  *     complete when that stream is synchronised.  Launches may be repeated
  *     (e.g. after new positions are written into the same buffers).
- *   genpk_batch_branch_of: the branch a job was assigned to. */
+ *   genpk_batch_branch_of: the branch a job was assigned to.
+ *   genpk_batch_plan_options: genpk_plan_set_options for the batch's plans
+ *     of one dims and precision (before genpk_batch_build). */
 typedef struct genpk_batch genpk_batch;
 int genpk_batch_create(genpk_batch **batch, int branches, void *stream);
 int genpk_batch_add(genpk_batch *batch, int64_t dims, int precision, double boxsize, int64_t n, const float *pos,
                     const float *masses, double mass, int nrbins, double total_mass, double *power, int *count,
                     double *keffs);
+int genpk_batch_plan_options(genpk_batch *batch, int64_t dims, int precision, int tile_x, int tile_y, int tile_z,
+                             int fft_variant);
 int genpk_batch_build(genpk_batch *batch, const int *branch_of_job);
 int genpk_batch_launch(genpk_batch *batch);
 int genpk_batch_branch_of(const genpk_batch *batch, int64_t job, int *branch);

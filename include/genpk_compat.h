@@ -35,6 +35,28 @@ int powerspectrum(int64_t dims, double (*outfield)[2], double (*outfield2)[2], i
  * grid [dims][dims][2(dims/2+1)], unnormalised, sign -1. */
 int genpk_fft_r2c_3d_inplace(int64_t dims, double *field);
 
+/* Resident mode of these shims (no counterpart in the reference; off by
+ * default, on when the environment holds GENPK_COMPAT_RESIDENT=1 or after
+ * genpk_compat_resident(1)).  It keeps the grid on the GPU across the
+ * reference driver's sequence on one host array -- zero it, fieldize once
+ * per chunk of particles (extra = 1), genpk_fft_r2c_3d_inplace, then
+ * powerspectrum of that array (gen-pk.cpp:208-234, read_fieldize.cpp:
+ * 43-94) -- so each forward moves the grid over PCIe once each way instead
+ * of twice per fieldize call:
+ *   - the first fieldize on an array loads it; later ones on the same array
+ *     add to the device copy, and the host array lacks their deposits until
+ *     the in-place FFT of that array (which writes the spectrum back, as
+ *     eager mode does) or genpk_compat_flush;
+ *   - powerspectrum of the array the last FFT wrote (as both fields) bins the
+ *     device copy of that spectrum: the caller must not change the host
+ *     array in between;
+ *   - any call on another array (or powerspectrum of two arrays) first
+ *     flushes, then runs as in eager mode.
+ * Turning it off flushes. */
+int genpk_compat_resident(int on);
+/* Write a resident density back to its host array. */
+int genpk_compat_flush(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,5 +1,5 @@
 """Gadget-HDF5 snapshot sets for tests, written and read back with the
-tests/helpers/hdf5_snap tool (built with the library by genpk_amd.build when
+tests/helpers/hdf5_snap tool (built by tests/helpers/build.py, from __graft_entry__.build(), when
 the HDF5 C library is present)."""
 import os
 import subprocess

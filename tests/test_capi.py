@@ -84,3 +84,20 @@ def test_print_pk_format(tmp_path):
     genpk_amd.print_pk(str(f), k, p, c)
     expect = "".join("%e\t%e\t%d\n" % (k[i], p[i], c[i]) for i in range(4) if c[i])
     assert f.read_text() == expect
+
+
+def test_radix_orderings():
+    """The radix orderings the FFT variants use (host only)."""
+    import genpk_amd
+    assert genpk_amd.radix_plan(1024, 0) == [8, 8, 8, 2]
+    assert genpk_amd.radix_plan(1024, 1) == [2, 8, 8, 8]
+    assert genpk_amd.radix_plan(1024, 2) == [4, 4, 4, 4, 4]
+    assert genpk_amd.radix_plan(384, 1) == [2, 3, 8, 8]
+    for n in (2, 6, 10, 48, 100, 192, 240, 384, 512, 1000):
+        for v in range(3):
+            r = genpk_amd.radix_plan(n, v)
+            assert int(np.prod(r)) == n and set(r) <= {2, 3, 4, 5, 8}
+            if v == 1:
+                assert sorted(r) == sorted(genpk_amd.radix_plan(n, 0))
+    with pytest.raises(genpk_amd.GenPKError):
+        genpk_amd.radix_plan(14, 0)

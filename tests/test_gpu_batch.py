@@ -43,8 +43,10 @@ def _jobs(torch):
     return jobs
 
 
-def _plain(job):
+def _plain(job, options=None):
     p = genpk_amd.Plan(job["dims"], job["precision"])
+    if options is not None:
+        p.set_options(options["tiles"], options["fft_variant"])
     p.clear()
     p.deposit(job["pos"], job["boxsize"], job["masses"])
     P, c, k = p.power(job["nrbins"], mass=job["total_mass"])
@@ -116,11 +118,15 @@ def test_batch_explicit_assignment_and_autotune(gpu, tmp_path, monkeypatch):
     b.close()
     best, rep = gb.tune(jobs, candidates=(1, 2), reps=2)
     assert rep["cached"] is False and rep["choice"]["branches"] in (1, 2)
+    assert set(rep["per_dims"]) == {f"{j['dims']}/{j['precision']}" for j in jobs}
     best.launch()
     torch.cuda.synchronize()
     for i, j in enumerate(jobs):
+        # the plain path with the options the tuner chose for this grid size
+        r = _plain(j, rep["per_dims"][f"{j['dims']}/{j['precision']}"])
         P = best.results(i)[0].cpu().numpy()
-        assert np.allclose(P, ref[i][0], rtol=1e-12, atol=0)
+        assert np.array_equal(best.results(i)[1].cpu().numpy(), r[1])
+        assert np.allclose(P, r[0], rtol=1e-12, atol=0)
     best.close()
     again, rep2 = gb.tune(jobs, candidates=(1, 2), reps=2)
     assert rep2["cached"] is True and rep2["choice"] == rep["choice"]
@@ -139,3 +145,57 @@ def test_batch_errors(gpu):
     with pytest.raises(genpk_amd.GenPKError):
         b.add(pos, -1.0, 16)
     b.close()
+
+
+@pytest.mark.timeout(600)
+def test_uneven_64_tuned_graph_vs_oracle(gpu, tmp_path, monkeypatch):
+    """The batched workload as the benchmark runs it: 64 realisations
+    (n_side log-uniform in [64, 256], dims uniform over {128, 256, 384, 512},
+    0.3-cell Gaussian lattices in a 100 Mpc/h box, FP32) through the
+    autotuned graph (deposit tile shape and FFT radix ordering per grid
+    size, then branches x policy).  Every job against the plain path with the
+    chosen options (counts and keffs equal, P within 1e-12); the smallest
+    job of each grid size against the FP64 CPU oracle: counts bit-exact, the
+    FP32 graph's P within 2e-4, and the same job through an FP64 plan with
+    the chosen options within 1e-8."""
+    import torch
+    import oracle
+    announce("uneven 64 realisations, tuned graph")
+    monkeypatch.setenv("GENPK_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    box = 100.0
+    jobs = []
+    for ns, d, sd in synth.uneven_realisations(64):
+        pos = synth.gaussian_lattice_torch(ns, box, d, 0.3, seed=sd)
+        jobs.append(dict(pos=pos, boxsize=box, dims=d, nrbins=d, masses=None, total_mass=float(pos.shape[0]),
+                         precision="f32"))
+    assert {j["dims"] for j in jobs} == {128, 256, 384, 512}
+    best, rep = gb.tune(jobs, reps=2, log=lambda m: print(m, flush=True))
+    print("per grid size:", {k: (v["tiles"], v["radices_d"]) for k, v in rep["per_dims"].items()},
+          "graph:", rep["choice"], flush=True)
+    for rep_i in range(2):
+        for out in best.results():
+            out[0].fill_(-1.0)
+        best.launch()
+        torch.cuda.synchronize()
+        for i, j in enumerate(jobs):
+            r = _plain(j, rep["per_dims"][f"{j['dims']}/f32"])
+            P, c, k = (t.cpu().numpy() for t in best.results(i))
+            assert np.array_equal(c, r[1]) and np.array_equal(k, r[2]), (rep_i, i)
+            assert np.allclose(P, r[0], rtol=1e-12, atol=0), (rep_i, i)
+    for d in (128, 256, 384, 512):
+        i = min((i for i, j in enumerate(jobs) if j["dims"] == d), key=lambda i: jobs[i]["pos"].shape[0])
+        j = jobs[i]
+        n = j["pos"].shape[0]
+        po, co, ko = oracle.pk_from_particles(box, d, j["pos"].cpu().numpy(), None, 1.0, d, float(n))
+        occ = co > 0
+        P, c, k = (t.cpu().numpy() for t in best.results(i))
+        assert np.array_equal(c, co), d
+        assert np.abs(k[occ] / ko[occ] - 1).max() <= 1e-12
+        e32 = np.abs(P[occ] / po[occ] - 1).max()
+        opts = rep["per_dims"][f"{d}/f32"]
+        P64 = _plain(dict(j, precision="f64"), opts)[0]
+        e64 = np.abs(P64[occ] / po[occ] - 1).max()
+        print(f"d={d} n={n} tiles {opts['tiles']} radices {opts['radices_d']}: f32 {e32:.2e} f64 {e64:.2e}",
+              flush=True)
+        assert e32 <= 2e-4 and e64 <= 1e-8, d
+    best.close()
