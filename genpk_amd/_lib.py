@@ -40,6 +40,7 @@ SIGNATURES = {
     "genpk_c2r": (_c_int, [_c_vp]),
     "genpk_bin": (_c_int, [_c_vp, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_dbl]),
     "genpk_power": (_c_int, [_c_vp, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_dbl]),
+    "genpk_power_noalloc": (_c_int, [_c_vp, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_dbl, _c_dbl]),
     "genpk_adjoint_density": (_c_int, [_c_vp, _c_int, _c_vp, _c_dbl]),
     "genpk_adjoint_gather": (_c_int, [_c_vp, _c_dbl, _c_i64, _c_vp, _c_vp, _c_dbl, _c_vp, _c_vp]),
     "genpk_batch_create": (_c_int, [ctypes.POINTER(_c_vp), _c_int, _c_vp]),

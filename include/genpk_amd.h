@@ -140,6 +140,20 @@ int genpk_bin(genpk_plan *a, genpk_plan *b, int nrbins, double *power_aa, double
 int genpk_power(genpk_plan *a, genpk_plan *b, int nrbins, double *power_aa, double *power_bb, double *power_ab,
                 int *count, double *keffs, double ma, double mb);
 
+/* genpk_power that allocates no device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, work and results as genpk_power; the per-workgroup partial bins
+ * live in a fixed 16 MiB device array of the library (one per GPU, held
+ * until the process ends) instead of a plan buffer.  Requirements: the bin
+ * tables of nrbins were built on plan a by an earlier genpk_power or
+ * genpk_bin (else an error), and the partial bins (resident workgroups x
+ * spectra x nrbins, e.g. 256 x 3 x 2730) fit in 2^21 doubles (else an error
+ * before any work: use fewer bins).  Calls that overlap on one GPU share the
+ * array and race. */
+int genpk_power_noalloc(genpk_plan *a, genpk_plan *b, int nrbins, double *power_aa, double *power_bb,
+                        double *power_ab, int *count, double *keffs, double ma, double mb);
+
 /* Adjoint of the auto spectrum (no counterpart in the reference).  The
  * plan must hold one of: the forward spectrum (genpk_r2c or
  * genpk_import_spectrum), the state left by genpk_power with nothing
