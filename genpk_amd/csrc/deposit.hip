@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const uint32_t *__restr
 // them) make the reference's eight CIC weights non-finite (fieldize.cpp:
 // 67-84: the fractional part is NaN or infinite), so their eight cells end
 // up NaN or infinite there.  Here those cells are set to NaN in the tile
-// store, at the cells cell_of gives (those of the reference on x86-64).
+// store, at the cells cell_of_x86 gives (those of the reference on x86-64).
 // The records are scanned only when k_deposit raised the flag.  A cell's
 // own store location is written (never a halo), so the order against the
 // halo folds does not matter, and later deposits keep it NaN.
@@ -309,9 +309,9 @@ __global__ void __launch_bounds__(256) k_poison(const float *__restrict__ rec, i
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const float *r = rec + i * rw;
     double fx, fy, fz;
-    const int cx = cell_of(r[0], kg.units, kg.d, fx);
-    const int cy = cell_of(r[1], kg.units, kg.d, fy);
-    const int cz = cell_of(r[2], kg.units, kg.d, fz);
+    const int cx = cell_of_x86(r[0], kg.units, kg.d, fx);
+    const int cy = cell_of_x86(r[1], kg.units, kg.d, fy);
+    const int cz = cell_of_x86(r[2], kg.units, kg.d, fz);
     if (fx + fy + fz <= 3.0) continue;
     const int nx = cx + 1 == kg.d ? 0 : cx + 1, ny = cy + 1 == kg.d ? 0 : cy + 1, nz = cz + 1 == kg.d ? 0 : cz + 1;
 #pragma unroll

@@ -17,12 +17,28 @@ KeyGeom key_geom(const Plan &p, double boxsize);
 
 // Floor cell of one coordinate, wrapped into [0, d), and the fractional part
 // (fieldize.cpp:66-75).  The floor is converted to a 32-bit integer as the
-// reference does; a floor outside the int range, or a non-finite position,
-// converts to INT_MIN, which is what that conversion gives on x86-64 (the
-// "integer indefinite" of cvttsd2si), so such a particle lands on the same
-// cells as there (its fractional part is then not finite).  Positions more
-// than one box outside [0, box) take the general modulo.
+// reference does; positions more than one box outside [0, box) take the
+// general modulo.  A floor outside the int range converts with the GPU's
+// saturation (NaN to 0), so every record still gets a valid tile; the
+// deposit skips a record whose fractional part is not finite, and k_poison
+// finds its cells with cell_of_x86.
 __device__ __forceinline__ int cell_of(float p, double units, int d, double &frac) {
+  const double u = (double)p * units;
+  const double f = floor(u);
+  frac = u - f;
+  int c = (int)f;
+  if ((unsigned)c >= (unsigned)d) {
+    c %= d;
+    if (c < 0) c += d;
+  }
+  return c;
+}
+
+// cell_of with the conversion x86-64 makes (cvttsd2si): a floor outside the
+// int range, NaN or an infinity converts to INT_MIN (the "integer
+// indefinite"), so a non-finite position maps to the cells the reference
+// gives it.
+__device__ __forceinline__ int cell_of_x86(float p, double units, int d, double &frac) {
   const double u = (double)p * units;
   const double f = floor(u);
   frac = u - f;

@@ -150,7 +150,8 @@ int genpk_power(genpk_plan *a, genpk_plan *b, int nrbins, double *power_aa, doub
  * genpk_bin (else an error), and the partial bins (resident workgroups x
  * spectra x nrbins, e.g. 256 x 3 x 2730) fit in 2^21 doubles (else an error
  * before any work: use fewer bins).  Calls that overlap on one GPU share the
- * array and race. */
+ * array and race.  (With stage timing enabled on the plan, the timer may
+ * create hipEvents; it allocates no device memory either.) */
 int genpk_power_noalloc(genpk_plan *a, genpk_plan *b, int nrbins, double *power_aa, double *power_bb,
                         double *power_ab, int *count, double *keffs, double ma, double mb);
 

@@ -329,3 +329,58 @@ def test_redeposit_and_stale_order(gpu, d):
     c.import_density(G)
     g_c = c.adjoint_gather(tp, box, tm)
     assert torch.equal(g_a[0], g_c[0]) and torch.equal(g_a[1], g_c[1])
+
+
+@pytest.mark.parametrize("prec,bar", [("f64", 1e-6), ("f32", 2e-3)])
+def test_adjoint_zeldovich_band_power_loss(gpu, prec, bar):
+    """The differentiable workload's own check on a 32^3 instance: 32^3
+    Zel'dovich particles (k^-2 displacement, 1-cell rms, seed 1) on a 32^3
+    grid with the cell size of 256^3 in 250 Mpc/h, masses 1 (a tensor), the
+    band-power loss against the P of a second seed (seed 2), sigma_b =
+    P_target,b sqrt(2/count_b).  Gradients w.r.t. positions and masses (recompute
+    "always") against the FP64 dense PyTorch reference, and, for the FP64 plan,
+    against central differences of the GPU loss on 32 coordinates at least
+    0.01 cell from a cell face."""
+    import torch
+    n_side, d = 32, 32
+    box = 250.0 * d / 256
+    announce("adjoint zeldovich band-power loss", prec, f"d={d} n={n_side}^3")
+    pos = synth.zeldovich(n_side, box, d, rms_cells=1.0, seed=1)
+    m = np.ones(pos.shape[0], dtype=np.float32)
+    tables = mode_tables(d, d)
+    ref_pos = synth.zeldovich(n_side, box, d, rms_cells=1.0, seed=2)
+    target, _, _, _ = _dense_grads(ref_pos, m, box, d, d, np.ones(d), tables)
+    Pd, Ld, gpd, gmd = _dense_grads(pos, m, box, d, d, target, tables)
+    plan = genpk_amd.Plan(d, prec)
+    Pg, Lg, gpg, gmg = _gpu_loss_and_grads(torch, pos, m, box, d, d, target, prec, plan, recompute="always")
+    assert abs(Lg / Ld - 1) <= (1e-8 if prec == "f64" else 1e-3)
+    okp, ep = _close(gpg, gpd, bar)
+    okm, em = _close(gmg, gmd, bar)
+    print(f"loss {Lg:.6e}; max rel grad err pos {ep:.3e} mass {em:.3e}", flush=True)
+    assert okp and okm
+    if prec != "f64":
+        return
+    cnt = torch.from_numpy(tables[2].numpy()).cuda()
+    tt = torch.from_numpy(target).cuda()
+    rng = np.random.default_rng(8)
+    checked = 0
+    with torch.no_grad():
+        for j in rng.permutation(pos.shape[0]):
+            a = int(rng.integers(3))
+            u = float(pos[j, a]) * d / box
+            if min(u - np.floor(u), np.ceil(u) - u) < 0.01 + 2e-4:
+                continue
+            h = np.float32(1e-4 * box / d)
+            ls = []
+            for sgn in (1, -1):
+                p2 = pos.copy()
+                p2[j, a] = np.float32(pos[j, a] + sgn * h)
+                P = power_spectrum(torch.from_numpy(p2).cuda(), box, d, d, masses=torch.from_numpy(m).cuda(),
+                                   total_mass=float(m.sum()), plan=plan)
+                ls.append((c3_loss(P, tt, cnt).item(), float(p2[j, a])))
+            fd = (ls[0][0] - ls[1][0]) / (ls[0][1] - ls[1][1])
+            assert abs(fd - gpg[j, a]) <= 1e-4 * np.abs(gpg).max() + 1e-3 * abs(gpg[j, a]), (j, a, fd, gpg[j, a])
+            checked += 1
+            if checked == 32:
+                break
+    assert checked == 32
