@@ -195,9 +195,9 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
       if (NF == 2) vb[NF == 2 ? q : 0] = ok ? fb[off + e * estride + b] : C{0, 0};
     }
   };
-  // REG1: block v + gridDim.x loads into the registers while block v is
-  // binned (their live range misses the FFT stages, where the registers peak)
-  if (REG1 && (int64_t)blockIdx.x < nblk) load(blockIdx.x);
+  // block v + gridDim.x loads into the registers while block v is binned
+  // (their live range misses the FFT stages, where the registers peak)
+  if ((int64_t)blockIdx.x < nblk) load(blockIdx.x);
   for (int64_t v = blockIdx.x; v < nblk; v += gridDim.x) {
     const int64_t blk = xcd_block((uint32_t)v, (uint32_t)nblk);
     const int64_t y = blk / nkb;
@@ -205,7 +205,6 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
     const int64_t kz0 = (int64_t)kb * B;
     const int nb = (int)min((int64_t)B, hz - kz0);
     const int64_t off = y * hz + kz0;
-    if (!REG1) load(v);
     C *res = buf;
     if constexpr (REG1) {
       fft_first_regs<MAXE, -1>(va, buf2, B, lgb, B, 1);
@@ -238,7 +237,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
         }
       }
     }
-    if (REG1 && v + gridDim.x < nblk) load(v + gridDim.x);
+    if (v + gridDim.x < nblk) load(v + gridDim.x);
     if (my_b < nb) {
       const int kz = (int)(kz0 + my_b);
       const int ky = (int)kval(y, d);

@@ -32,11 +32,12 @@ constexpr int FOLD_THREADS = 256;
 
 // 2^36 * (a * b) rounded to nearest once, for a in [0, 1] and b = (a weight
 // in [0, 1]) * 2^36: the integer lands in the low mantissa bits of
-// a * b + 2^52.  bias = 2^52 comes from the kernel arguments, so it stays in
-// a scalar register and the product is one FMA.
+// a * b + 2^52, which lies in [2^52, 2^53), so masking off the exponent
+// leaves it (one AND on the high word).  bias = 2^52 comes from the kernel
+// arguments, so it stays in a scalar register and the product is one FMA.
 __device__ __forceinline__ unsigned long long to_fixed(double a, double b, double bias) {
   const double t = fma(a, b, bias);
-  return (unsigned long long)__double_as_longlong(t) - 0x4330000000000000ull;
+  return (unsigned long long)__double_as_longlong(t) & 0x000fffffffffffffull;
 }
 
 // round(a * b) as a two's complement u64 for |a * b| < 2^51: a * b + 1.5 * 2^52

@@ -162,6 +162,7 @@ struct Plan {
   DevBuf recA, recB, table1, table2, seginfo, tile_start, item_start, part_start, heavy, partials, item_tile;
   DevBuf permA, permB;      // sort permutations kept for the adjoint gather
   DevBuf dig2;              // low tile-key digit of each pass-1 output record (one byte)
+  DevBuf pages;             // paged first sort pass: block digits, fills and lists
   DevBuf mrange;            // largest and smallest nonzero |mass| of the last sort (float bits)
   DevBuf nonfinite;         // flag of the last deposit: a particle position is not finite
   DevBuf scan_tmp;

@@ -7,6 +7,8 @@
 // (the tile kernels do not depend on it).
 #include "tilesort.h"
 
+#include <algorithm>
+
 namespace genpk {
 
 namespace {
@@ -88,6 +90,26 @@ struct ChunkMap {
     ts = (int64_t)chunk_start[s + 1] - c0;
     return b < e;
   }
+};
+
+// Paged pass 1 (MODE 1 and 2 of k_downsweep, two-pass sorts of many
+// records): the pass-1 output pool is cut into blocks of PG_BLOCK record
+// slots, Q per pass-1 workgroup.
+constexpr int PG_LOG = 10;
+constexpr uint32_t PG_BLOCK = 1u << PG_LOG;
+
+// A block-list entry: the block and its fill (1 to PG_BLOCK records) in one
+// word, so that one load gives both.
+__host__ __device__ constexpr uint32_t pg_entry(uint32_t blk, uint32_t fill) { return blk | (fill - 1) << 22; }
+__device__ __forceinline__ uint32_t pg_blk(uint32_t e) { return e & ((1u << 22) - 1); }
+__device__ __forceinline__ uint32_t pg_fill(uint32_t e) { return (e >> 22) + 1; }
+
+struct PgArgs {
+  uint32_t *hdr = nullptr;          // [G * Q] of each taken block: digit | (its number among the digit's blocks) << 8
+  uint32_t *fill = nullptr;         // [G * Q] records in each taken block
+  uint32_t *used = nullptr;         // [G] blocks taken by each pass-1 workgroup
+  int64_t Q = 0;                    // blocks per pass-1 workgroup
+  const uint32_t *blist = nullptr;  // pass 2: the taken blocks, segment (pass-1 digit) by segment (pg_entry)
 };
 
 // Tile key of a record with the cells along the axes not in `axes` (bit a:
@@ -235,7 +257,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_upsweep_bytes(const uint8_t *_
 // With `two`, lo also receives each record's digit of ds2 (computed from the
 // same cells).
 template <int NM, int PL>
-__device__ __forceinline__ void rank_records(const float (&rec)[PL][Src<NM>::RW], int wbase, int nsub,
+__device__ __forceinline__ void rank_records(const float (&rec)[PL][Src<NM>::RW], const bool (&vld)[PL],
                                              const KeyGeom &kg, const DigitSel &ds, const DigitSel &ds2, bool two,
                                              uint32_t *wrow, uint32_t (&dig)[PL], uint32_t (&lo)[PL],
                                              uint32_t (&rnk)[PL]) {
@@ -244,7 +266,7 @@ __device__ __forceinline__ void rank_records(const float (&rec)[PL][Src<NM>::RW]
   int glead[PL];
 #pragma unroll
   for (int it = 0; it < PL; it++) {
-    const bool valid = wbase + it * 64 < nsub;
+    const bool valid = vld[it];
     const uint32_t key = valid ? key_on_axes(rec[it], kg, two ? ds.axes | ds2.axes : ds.axes) : 0u;
     const uint32_t dg = ds.of_key(key);
     lo[it] = ds2.of_key(key);
@@ -306,34 +328,52 @@ __device__ __forceinline__ uint32_t block_scan_round(uint32_t v, uint32_t (*wtot
   return run;
 }
 
-// Partition of each chunk by digit into dst at offsets[tb + dg * ts].  perm (optional): perm[i] = output position of
-// input record i.  Per sub-chunk of SORT_THREADS * PL records (each wave
-// owns 64 * PL consecutive ones): ranks (rank_records), then one thread per
-// digit turns the per-wave counts into offsets and a block scan gives the
-// digit's start in a digit-sorted staging copy and dbase = (global output
-// offset - staging start) mod 2^32, records go to the staging copy, and the
-// staging copy is written out as contiguous runs (record p goes to
+// Partition of each chunk by digit.  perm (optional): perm[i] = output
+// position of input record i.  Per sub-chunk of SORT_THREADS * PL records
+// (each wave owns 64 * PL consecutive ones): ranks (rank_records), then one
+// thread per digit turns the per-wave counts into offsets and a block scan
+// gives the digit's start in a digit-sorted staging copy and dbase = (global
+// output offset - staging start) mod 2^32, records go to the staging copy,
+// and the staging copy is written out as contiguous runs (record p goes to
 // dbase[its digit] + p) while the next sub-chunk's records load.  With DB
 // the per-wave counts are double-buffered: this sub-chunk's buffer is
 // cleared during the write-out and the next sub-chunk ranks into the other
 // one, so no barrier separates them.  hw count rows (a divisor of the wave
 // count) serve the waves, wave w on row w % hw.
-template <int NM, int PL, bool DB>
+//   MODE 0: the chunk's output offsets per digit are offsets[tb + dg * ts].
+//   MODE 1 (paged pass 1): workgroup g appends each digit's records to
+//     blocks of PG_BLOCK slots that it takes in order from its own range
+//     [g Q, g Q + Q) of the pool: the digit's current block until it is
+//     full, then new ones (a run then has two parts, split at gbase[dg]).
+//     Each block's digit and number among the digit's blocks go to pg.hdr,
+//     its fill to pg.fill, the number of blocks taken to pg.used[g].  No
+//     histogram pass and no global atomics.
+//   MODE 2 (paged pass 2): the chunk covers list slots of its segment's
+//     blocks (pg.blist); slot v is record (block of blist[v / PG_BLOCK], v %
+//     PG_BLOCK) when below that block's fill, and is skipped otherwise.
+//     perm is indexed by pool slot.
+template <int NM, int PL, bool DB, int MODE>
 __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, float *__restrict__ dst, ChunkMap cm,
                                                                KeyGeom kg, DigitSel ds, int nb, int hw,
                                                                const uint32_t *__restrict__ offsets,
                                                                uint32_t *__restrict__ perm,
                                                                uint8_t *__restrict__ dig2, DigitSel ds2,
-                                                               uint32_t *__restrict__ mrange) {
+                                                               uint32_t *__restrict__ mrange, PgArgs pg) {
   constexpr int RW = Src<NM>::RW;
   constexpr int SUBN = SORT_THREADS * PL;
   constexpr int NBUF = DB ? 2 : 1;
+  constexpr uint32_t NONE = 0xffffffffu;
+  static_assert(MODE != 2 || PL * 64 <= (int)PG_BLOCK, "a wave's records of a sub-chunk lie in one block");
   extern __shared__ uint32_t sm[];
   uint32_t *whist_all = sm;                                  // [NBUF][hw][nb]
   uint32_t *lstart = whist_all + NBUF * hw * nb;             // [nb]
-  uint32_t *gbase = lstart + nb;                             // [nb]
+  uint32_t *gbase = lstart + nb;                             // [nb] (MODE 1: first staging position of a run's second part)
   uint32_t *dbase = gbase + nb;                              // [nb]
-  uint16_t *sdig = reinterpret_cast<uint16_t *>(dbase + nb);  // [SUBN]
+  uint32_t *dbase2 = dbase + nb;                             // MODE 1 [nb]: dbase of a run's second part
+  uint32_t *cur_blk = dbase2 + (MODE == 1 ? nb : 0);         // MODE 1 [nb]: the digit's current block
+  uint32_t *cur_fill = cur_blk + (MODE == 1 ? nb : 0);       // MODE 1 [nb]: its fill (PG_BLOCK: none open)
+  uint32_t *dblocks = cur_fill + (MODE == 1 ? nb : 0);       // MODE 1 [nb]: blocks the digit took
+  uint16_t *sdig = reinterpret_cast<uint16_t *>(dblocks + (MODE == 1 ? nb : 0));  // [SUBN]
   float *stage = reinterpret_cast<float *>(sdig + SUBN);      // [SUBN][RW]
   __shared__ uint32_t wtot[2][SORT_WAVES];
   __shared__ uint32_t smr[2];  // masses: largest, smallest nonzero |mass| bits of this chunk
@@ -343,7 +383,16 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // with dig2 (digits of at most 8 bits), sdig holds the digit and the ds2 digit
   const bool two = dig2 != nullptr;
-  for (int i = threadIdx.x; i < nb; i += SORT_THREADS) gbase[i] = offsets[tb + (int64_t)i * ts];
+  if (MODE != 1)
+    for (int i = threadIdx.x; i < nb; i += SORT_THREADS) gbase[i] = offsets[tb + (int64_t)i * ts];
+  if (MODE == 1)
+    for (int i = threadIdx.x; i < nb; i += SORT_THREADS) {
+      cur_blk[i] = NONE;
+      cur_fill[i] = PG_BLOCK;
+      dblocks[i] = 0;
+    }
+  const uint32_t blk0 = MODE == 1 ? (uint32_t)(blockIdx.x * pg.Q) : 0u;  // this workgroup's block range
+  uint32_t taken = 0;                                                    // MODE 1: blocks taken so far
   for (int i = threadIdx.x; i < NBUF * hw * nb; i += SORT_THREADS) whist_all[i] = 0;
   const int row = (wave % hw) * nb;
   if (threadIdx.x == 0) {
@@ -353,40 +402,104 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
   uint32_t mx = 0u, mn = 0xffffffffu;
 
   float rec[PL][RW];
+  bool vld[PL];  // record it of this lane is valid
   const int wbase = wave * (PL * 64) + lane;
+  // MODE 2: a wave's 64 * PL consecutive list slots (PL * 64 <= PG_BLOCK,
+  // sub-chunks start at multiples of PG_BLOCK) lie in one block: its pool
+  // offset and fill are looked up once per wave and sub-chunk
+  uint32_t wblk = 0, wfill = 0;
+  // records of the sub-chunk at s0 (MODE 2: through the block list)
+  auto load_sub = [&](int64_t s0) {
+    if constexpr (MODE == 2) {
+      const int64_t i0 = s0 + wave * (PL * 64);
+      wfill = 0;
+      if (i0 < ce) {
+        const uint32_t ent = __builtin_amdgcn_readfirstlane(pg.blist[i0 >> PG_LOG]);
+        wblk = pg_blk(ent);
+        wfill = pg_fill(ent);
+      }
+    }
 #pragma unroll
-  for (int it = 0; it < PL; it++) {
-    const int64_t i = cb + wbase + it * 64;
-    if (i < ce) src.load(i, rec[it]);
-  }
+    for (int it = 0; it < PL; it++) {
+      const int64_t i = s0 + wbase + it * 64;
+      if constexpr (MODE == 2) {
+        const uint32_t off = (uint32_t)i & (PG_BLOCK - 1);
+        vld[it] = i < ce && off < wfill;
+        if (vld[it]) src.load(wblk * PG_BLOCK + off, rec[it]);
+      } else {
+        vld[it] = i < ce;
+        if (vld[it]) src.load(i, rec[it]);
+      }
+    }
+  };
+  load_sub(cb);
   __syncthreads();
 
   int sub = 0;
   for (int64_t s0 = cb; s0 < ce; s0 += SUBN, sub++) {
-    const int nsub = (int)min((int64_t)SUBN, ce - s0);
     uint32_t *whist = whist_all + (DB ? (sub & 1) * hw * nb : 0);
     uint32_t dig[PL], lo[PL], rnk[PL];
-    rank_records<NM, PL>(rec, wbase, nsub, kg, ds, ds2, two, whist + row, dig, lo, rnk);
+    rank_records<NM, PL>(rec, vld, kg, ds, ds2, two, whist + row, dig, lo, rnk);
     __syncthreads();
+    uint32_t total = 0;  // records of this sub-chunk
     {
-      uint32_t total = 0;
       for (int r0 = 0, rd = 0; r0 < nb; r0 += SORT_THREADS, rd++) {
         const int dg = r0 + threadIdx.x;
         const uint32_t tot = dg < nb ? wave_offsets(whist, nb, hw, dg) : 0u;
-        const uint32_t run = block_scan_round(tot, wtot, rd, total);
-        if (dg < nb) {
-          const uint32_t g = gbase[dg];
-          lstart[dg] = run;
-          dbase[dg] = g - run;
-          gbase[dg] = g + tot;
+        if constexpr (MODE == 1) {
+          // the run fills the current block (a records), then `need` new
+          // blocks; counts and block numbers scan together (nb <= 256, one
+          // round: at most SUBN records and SUBN / PG_BLOCK + nb blocks)
+          const uint32_t cf = dg < nb ? cur_fill[dg] : PG_BLOCK;
+          const uint32_t a = min(tot, PG_BLOCK - cf);
+          const uint32_t need = (tot - a + PG_BLOCK - 1) >> PG_LOG;
+          const uint32_t run = block_scan_round(tot | need << 16, wtot, rd, total);
+          if (dg < nb) {
+            const uint32_t lr = run & 0xffffu, first = blk0 + taken + (run >> 16);
+            const uint32_t cbk = cur_blk[dg];
+            lstart[dg] = lr;
+            dbase[dg] = cbk * PG_BLOCK + cf - lr;  // used only when a > 0, i.e. cbk is open
+            gbase[dg] = lr + a;
+            dbase2[dg] = first * PG_BLOCK - (lr + a);
+            if (need) {
+              if (cbk != NONE) pg.fill[cbk] = PG_BLOCK;  // filled by this run
+              const uint32_t nd = dblocks[dg];
+              for (uint32_t k = 0; k < need; k++) {
+                pg.hdr[first + k] = (uint32_t)dg | (nd + k) << 8;
+                if (k + 1 < need) pg.fill[first + k] = PG_BLOCK;
+              }
+              dblocks[dg] = nd + need;
+              cur_blk[dg] = first + need - 1;
+              cur_fill[dg] = tot - a - (need - 1) * PG_BLOCK;
+            } else {
+              cur_fill[dg] = cf + tot;
+            }
+          }
+        } else {
+          const uint32_t run = block_scan_round(tot, wtot, rd, total);
+          if (dg < nb) {
+            const uint32_t g = gbase[dg];
+            lstart[dg] = run;
+            dbase[dg] = g - run;
+            gbase[dg] = g + tot;
+          }
         }
+      }
+      if constexpr (MODE == 1) {
+        taken += total >> 16;
+        total &= 0xffffu;
       }
     }
     __syncthreads();
+    // output position of the record at staging position p of digit d
+    auto dest_of = [&](uint32_t p, uint32_t d) -> uint32_t {
+      if constexpr (MODE == 1) return p < gbase[d] ? dbase[d] + p : dbase2[d] + p;
+      else return dbase[d] + p;
+    };
 #pragma unroll
     for (int it = 0; it < PL; it++) {
       const int loc = wbase + it * 64;
-      if (loc < nsub) {
+      if (vld[it]) {
         const uint32_t dg = dig[it];
         const uint32_t p = lstart[dg] + whist[row + dg] + rnk[it];
 #pragma unroll
@@ -398,22 +511,16 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
           mx = max(mx, mb);
           if (mb) mn = min(mn, mb);
         }
-        if (perm) perm[s0 + loc] = dbase[dg] + p;  // where this input record goes (coalesced)
+        // where this record goes (coalesced writes, by input position or pool slot)
+        if (perm) perm[MODE == 2 ? (int64_t)(wblk * PG_BLOCK + ((uint32_t)(s0 + loc) & (PG_BLOCK - 1))) : s0 + loc] = dest_of(p, dg);
       }
     }
     // the next sub-chunk loads during the barrier and the write-out
-    {
-      const int64_t n0 = s0 + SUBN;
-#pragma unroll
-      for (int it = 0; it < PL; it++) {
-        const int64_t i = n0 + wbase + it * 64;
-        if (i < ce) src.load(i, rec[it]);
-      }
-    }
+    load_sub(s0 + SUBN);
     __syncthreads();
     auto put = [&](int p) {
       const uint32_t sd = sdig[p];
-      const uint32_t dest = dbase[two ? sd & 0xffu : sd] + (uint32_t)p;
+      const uint32_t dest = dest_of((uint32_t)p, two ? sd & 0xffu : sd);
       float v[RW];
 #pragma unroll
       for (int k = 0; k < RW; k++) v[k] = stage[p * RW + k];
@@ -421,14 +528,26 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
       for (int k = 0; k < RW; k++) dst[(int64_t)dest * RW + k] = v[k];
       if (two) dig2[dest] = (uint8_t)(sd >> 8);
     };
-    if (nsub == SUBN) {
+    if (total == SUBN) {
+      if constexpr (MODE == 1) {
+        // two-part runs: fewer writes in flight keep the registers in bounds
+#pragma unroll 2
+        for (int it = 0; it < PL; it++) put(threadIdx.x + it * SORT_THREADS);
+      } else {
 #pragma unroll
-      for (int it = 0; it < PL; it++) put(threadIdx.x + it * SORT_THREADS);
+        for (int it = 0; it < PL; it++) put(threadIdx.x + it * SORT_THREADS);
+      }
     } else {
-      for (int p = threadIdx.x; p < nsub; p += SORT_THREADS) put(p);
+      for (int p = threadIdx.x; p < (int)total; p += SORT_THREADS) put(p);
     }
     for (int i = threadIdx.x; i < hw * nb; i += SORT_THREADS) whist[i] = 0;
     if (!DB) __syncthreads();
+  }
+  if constexpr (MODE == 1) {
+    // the open blocks' fills (each digit's state is its own thread's)
+    for (int dg = threadIdx.x; dg < nb; dg += SORT_THREADS)
+      if (cur_blk[dg] != NONE) pg.fill[cur_blk[dg]] = cur_fill[dg];
+    if (threadIdx.x == 0) pg.used[blockIdx.x] = taken;
   }
   if constexpr (NM == 1) {
     if (mrange) {
@@ -465,9 +584,10 @@ __global__ void k_seginfo(const uint32_t *__restrict__ t1scan, int64_t g1, int n
 }
 
 // Two-pass tile starts from the scanned pass-2 table: the offset of a tile's
-// first chunk, or the segment start when the segment is empty; tile_start[ks] = n.
+// first chunk, or the segment's first record (rec_start) when the segment is
+// empty; tile_start[ks] = n.
 __global__ void k_tile_start_2pass(const uint32_t *__restrict__ t2scan, const uint32_t *__restrict__ chunk_start,
-                                   const uint32_t *__restrict__ seg_start, int nhi, int nlo, int64_t n,
+                                   const uint32_t *__restrict__ rec_start, int nhi, int nlo, int64_t n,
                                    uint32_t *__restrict__ tile_start) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t ks = (int64_t)nhi * nlo;
@@ -478,7 +598,134 @@ __global__ void k_tile_start_2pass(const uint32_t *__restrict__ t2scan, const ui
   }
   const int hi = (int)(t / nlo), lo = (int)(t % nlo);
   const int64_t c0 = chunk_start[hi], nc = (int64_t)chunk_start[hi + 1] - c0;
-  tile_start[t] = nc ? t2scan[c0 * nlo + lo * nc] : seg_start[hi];
+  tile_start[t] = nc ? t2scan[c0 * nlo + lo * nc] : rec_start[hi];
+}
+
+// Paged pass 1's block directory.  Per pass-1 workgroup w (one workgroup
+// each): the blocks and records of each digit among the blocks it took, at
+// bcnt / rcnt[d * G + w]; entry nhi * G is zero, so that exclusive scans of
+// the nhi * G + 1 entries end in the totals.
+__global__ void __launch_bounds__(256) k_pg_count(PgArgs pg, int nhi, int64_t G, uint32_t *__restrict__ bcnt,
+                                                  uint32_t *__restrict__ rcnt) {
+  __shared__ uint32_t cb[256], cr[256];
+  const int64_t w = blockIdx.x;
+  for (int i = threadIdx.x; i < nhi; i += 256) cb[i] = cr[i] = 0;
+  __syncthreads();
+  const uint32_t used = (uint32_t)min((int64_t)pg.used[w], pg.Q);
+  const int64_t b0 = w * pg.Q;
+  for (uint32_t k = threadIdx.x; k < used; k += 256) {
+    const uint32_t d = pg.hdr[b0 + k] & 0xffu;
+    if (d < (uint32_t)nhi) {
+      atomicAdd(&cb[d], 1u);
+      atomicAdd(&cr[d], pg.fill[b0 + k]);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nhi; i += 256) {
+    bcnt[(int64_t)i * G + w] = cb[i];
+    rcnt[(int64_t)i * G + w] = cr[i];
+  }
+  if (w == 0 && threadIdx.x == 0) {
+    bcnt[(int64_t)nhi * G] = 0;
+    rcnt[(int64_t)nhi * G] = 0;
+  }
+}
+
+// The block list: digit d's blocks from pass-1 workgroup w at boff[d * G +
+// w] onwards, in the order the workgroup took them (each block's number
+// among its digit's blocks is in its header), so a segment lists its records
+// in input order, as the plain pass 1 writes them.
+__global__ void __launch_bounds__(256) k_pg_list(PgArgs pg, int nhi, int64_t G, const uint32_t *__restrict__ boff,
+                                                 uint32_t *__restrict__ blist) {
+  __shared__ uint32_t at[256], cnt[256];
+  const int64_t w = blockIdx.x;
+  for (int i = threadIdx.x; i < nhi; i += 256) {
+    at[i] = boff[(int64_t)i * G + w];
+    cnt[i] = boff[(int64_t)i * G + w + 1] - at[i];
+  }
+  __syncthreads();
+  const uint32_t used = (uint32_t)min((int64_t)pg.used[w], pg.Q);
+  const int64_t b0 = w * pg.Q;
+  for (uint32_t k = threadIdx.x; k < used; k += 256) {
+    const uint32_t h = pg.hdr[b0 + k], d = h & 0xffu;
+    if (d < (uint32_t)nhi && (h >> 8) < cnt[d]) blist[at[d] + (h >> 8)] = pg_entry((uint32_t)(b0 + k), pg.fill[b0 + k]);
+  }
+}
+
+// Pass-2 segments over the block list: segment s (pass-1 digit s) covers
+// list slots [seg_start[s], seg_start[s+1]) (PG_BLOCK per listed block), its
+// records start at rec_start[s] of the output, and it has nchunks[s] chunks
+// of `chunk` list slots.
+__global__ void k_pg_seginfo(const uint32_t *__restrict__ boff, const uint32_t *__restrict__ roff, int nhi,
+                             int64_t G, int64_t chunk, uint32_t *__restrict__ seg_start,
+                             uint32_t *__restrict__ rec_start, uint32_t *__restrict__ nchunks) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > nhi) return;
+  const uint32_t b0 = boff[(int64_t)s * G];
+  seg_start[s] = b0 * PG_BLOCK;
+  rec_start[s] = roff[(int64_t)s * G];
+  nchunks[s] = s < nhi ? (uint32_t)(((int64_t)(boff[(int64_t)(s + 1) * G] - b0) * PG_BLOCK + chunk - 1) / chunk) : 0u;
+}
+
+// k_upsweep_bytes over the list slots of a paged pass 2: the digit bytes of
+// the listed blocks' filled slots.  Chunks start at multiples of PG_BLOCK
+// list slots; each wave takes whole blocks (one lookup of the block and its
+// fill), each lane 16 consecutive slots of it.
+__global__ void __launch_bounds__(SORT_THREADS) k_upsweep_pages(const uint8_t *__restrict__ digs, ChunkMap cm,
+                                                                PgArgs pg, int nb, uint32_t *__restrict__ counts) {
+  static_assert(PG_BLOCK == 64 * 16, "one block per wave, 16 slots per lane");
+  extern __shared__ uint32_t hist[];
+  int64_t b, e, tb, ts;
+  if (!cm.range(blockIdx.x, b, e, tb, ts)) return;
+  for (int i = threadIdx.x; i < nb; i += SORT_THREADS) hist[i] = 0;
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t k0 = b >> PG_LOG, k1 = (e + PG_BLOCK - 1) >> PG_LOG;  // list entries of the chunk
+  constexpr int UNR = 4;  // blocks per wave in flight
+  for (int64_t kb = k0 + wave; kb < k1; kb += UNR * SORT_WAVES) {
+    uint32_t blk[UNR];
+    int nv[UNR];
+    uint4 v[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      const int64_t k = kb + u * SORT_WAVES;
+      const uint32_t ent = k < k1 ? __builtin_amdgcn_readfirstlane(pg.blist[k]) : 0u;
+      blk[u] = pg_blk(ent);
+      // filled slots of this block inside the chunk, from this lane's 16
+      const int64_t lim = k < k1 ? min((int64_t)pg_fill(ent), e - (k << PG_LOG)) : 0;
+      nv[u] = (int)max((int64_t)0, min((int64_t)16, lim - 16 * lane));
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      v[u] = make_uint4(0u, 0u, 0u, 0u);
+      if (nv[u] > 0) v[u] = *reinterpret_cast<const uint4 *>(digs + (int64_t)blk[u] * PG_BLOCK + 16 * lane);
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; u++) {
+      // 16 equal digits add with one atomic, and a wavefront of such lanes
+      // of one digit with one atomic for all
+      const uint32_t d0 = v[u].x & 0xffu;
+      const uint32_t rep = d0 * 0x01010101u;
+      const bool same16 = nv[u] == 16 && v[u].x == rep && v[u].y == rep && v[u].z == rep && v[u].w == rep;
+      const uint32_t dw = __builtin_amdgcn_readfirstlane(d0);
+      if (__ballot(same16 && d0 == dw) == ~0ull) {
+        if (lane == 0) atomicAdd(&hist[dw], 1024u);
+      } else if (same16) {
+        atomicAdd(&hist[d0], 16u);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const uint32_t wq = q == 0 ? v[u].x : q == 1 ? v[u].y : q == 2 ? v[u].z : v[u].w;
+#pragma unroll
+          for (int j = 0; j < 4; j++)
+            if (4 * q + j < nv[u]) atomicAdd(&hist[(wq >> (8 * j)) & 0xffu], 1u);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb; i += SORT_THREADS)
+    counts[tb + (int64_t)i * ts] = hist[i];
 }
 
 // Single-pass tile starts: tile_start[k] = t1scan[k*g1] (chunk 0), tile_start[ks] = n
@@ -528,60 +775,102 @@ struct DsShape {
   int hw;
 };
 
-size_t downsweep_lds(int nb, int rw, const DsShape &h) {
+// LDS of a downsweep shape; MODE 1 keeps four more arrays per digit.
+size_t downsweep_lds(int nb, int rw, const DsShape &h, int mode = 0) {
   const size_t sub = (size_t)SORT_THREADS * h.pl;
-  return (size_t)((h.db ? 2 : 1) * h.hw * nb + 3 * nb) * 4 + sub * 2 + sub * rw * 4;
+  return (size_t)((h.db ? 2 : 1) * h.hw * nb + (mode == 1 ? 7 : 3) * nb) * 4 + sub * 2 + sub * rw * 4;
 }
 
 // The first shape whose LDS fits one workgroup per CU: four count rows (each
 // shared by four waves: 16 rows measured 0.1 ms slower per downsweep at 2^30
 // records, the per-digit offsets running over more rows), then the most
 // records per lane, then double-buffered counts.
-DsShape downsweep_shape(int nb, int rw) {
+DsShape downsweep_shape(int nb, int rw, int mode = 0) {
   const size_t cap = 160 * 1024 - 256;
   for (int hw : {SORT_WAVES / 4, 1})
     for (int pl : {8, 4, 2})
       for (bool db : {true, false}) {
         const DsShape h{pl, db, hw};
-        if (downsweep_lds(nb, rw, h) <= cap) return h;
+        if (downsweep_lds(nb, rw, h, mode) <= cap) return h;
       }
   return DsShape{1, false, 1};
 }
 
-template <int NM>
+// MODE 1 and 2 run with 8 records per lane only (sub-chunks of 8192 records)
+template <int NM, int MODE = 0>
 void launch_downsweep(int64_t grid, hipStream_t s, Src<NM> src, float *dst, const ChunkMap &cm, const KeyGeom &kg,
                       const DigitSel &ds, int nb, const uint32_t *offsets, uint32_t *perm,
                       uint8_t *dig2 = nullptr, const DigitSel &ds2 = DigitSel{0, 0, 0},
-                      uint32_t *mrange = nullptr) {
+                      uint32_t *mrange = nullptr, const PgArgs &pg = PgArgs{}) {
   GENPK_REQUIRE(nb <= 2 * SORT_THREADS, "sort digit too wide");
   GENPK_REQUIRE(!dig2 || (nb <= 256 && ds2.mask <= 255u), "sort digit bytes need digits of at most 8 bits");
-  const DsShape h = downsweep_shape(nb, Src<NM>::RW);
-  const size_t lds = downsweep_lds(nb, Src<NM>::RW, h);
+  const DsShape h = downsweep_shape(nb, Src<NM>::RW, MODE);
+  const size_t lds = downsweep_lds(nb, Src<NM>::RW, h, MODE);
   const dim3 g((unsigned)grid), b(SORT_THREADS);
-  if (h.pl == 8 && h.db)
-    hipLaunchKernelGGL((k_downsweep<NM, 8, true>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
-                       dig2, ds2, mrange);
-  else if (h.pl == 8)
-    hipLaunchKernelGGL((k_downsweep<NM, 8, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
-                       dig2, ds2, mrange);
-  else if (h.pl == 4 && h.db)
-    hipLaunchKernelGGL((k_downsweep<NM, 4, true>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
-                       dig2, ds2, mrange);
-  else if (h.pl == 4)
-    hipLaunchKernelGGL((k_downsweep<NM, 4, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
-                       dig2, ds2, mrange);
-  else if (h.pl == 2)
-    hipLaunchKernelGGL((k_downsweep<NM, 2, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
-                       dig2, ds2, mrange);
-  else
-    hipLaunchKernelGGL((k_downsweep<NM, 1, false>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
-                       dig2, ds2, mrange);
+  if constexpr (MODE != 0) {
+    GENPK_REQUIRE(h.pl == 8 && nb <= 256 && (MODE != 1 || dig2), "paged sort pass: unsupported shape");
+    if (h.db)
+      hipLaunchKernelGGL((k_downsweep<NM, 8, true, MODE>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets,
+                         perm, dig2, ds2, mrange, pg);
+    else
+      hipLaunchKernelGGL((k_downsweep<NM, 8, false, MODE>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets,
+                         perm, dig2, ds2, mrange, pg);
+  } else if (h.pl == 8 && h.db) {
+    hipLaunchKernelGGL((k_downsweep<NM, 8, true, 0>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
+                       dig2, ds2, mrange, pg);
+  } else if (h.pl == 8) {
+    hipLaunchKernelGGL((k_downsweep<NM, 8, false, 0>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
+                       dig2, ds2, mrange, pg);
+  } else if (h.pl == 4 && h.db) {
+    hipLaunchKernelGGL((k_downsweep<NM, 4, true, 0>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
+                       dig2, ds2, mrange, pg);
+  } else if (h.pl == 4) {
+    hipLaunchKernelGGL((k_downsweep<NM, 4, false, 0>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
+                       dig2, ds2, mrange, pg);
+  } else if (h.pl == 2) {
+    hipLaunchKernelGGL((k_downsweep<NM, 2, false, 0>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
+                       dig2, ds2, mrange, pg);
+  } else {
+    hipLaunchKernelGGL((k_downsweep<NM, 1, false, 0>), g, b, lds, s, src, dst, cm, kg, ds, nb, h.hw, offsets, perm,
+                       dig2, ds2, mrange, pg);
+  }
   GENPK_LAUNCH_CHECK();
+}
+
+// Whether and how a sort runs its first pass paged: records without masses,
+// two passes with digits of at most 8 bits, and so many records that the
+// partly filled blocks (at most one per pass-1 workgroup and digit) stay
+// below 1/8 of them (half of that on average).  Each of the G = CU-count workgroups takes `chunk`
+// records (whole sub-chunks) and Q = chunk / PG_BLOCK + nhi blocks: every
+// digit's blocks but its last are full.
+struct PgLayout {
+  bool on = false;
+  int64_t G = 0, chunk = 0, Q = 0;
+  int64_t slots = 0;  // G * Q * PG_BLOCK record slots of the pass-1 output
+};
+
+PgLayout pg_layout(const Plan &p, int64_t n, int rw) {
+  PgLayout L;
+  const TileGeom &tg = p.tg;
+  // records of positions only: with masses (16-byte records) the paged
+  // downsweeps would spill registers
+  if (rw != 3 || tg.passes != 2 || tg.hi_bits > 8 || tg.lo_bits > 8) return L;
+  const int64_t nhi = (int64_t)1 << tg.hi_bits, cus = num_cus();
+  if (cus * nhi * (int64_t)PG_BLOCK > n / 8) return L;
+  for (int mode : {1, 2})
+    if (downsweep_shape((int)nhi, rw, mode).pl != 8) return L;
+  constexpr int64_t SUBN = (int64_t)SORT_THREADS * 8;
+  L.chunk = ceil_div(ceil_div(n, cus), SUBN) * SUBN;
+  L.G = ceil_div(n, L.chunk);
+  L.Q = L.chunk / PG_BLOCK + nhi;
+  L.slots = L.G * L.Q * PG_BLOCK;
+  L.on = L.slots <= ((int64_t)1 << 32);  // block numbers below 2^22 (pg_entry)
+  return L;
 }
 
 template <int NM>
 TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, const float *masses, bool with_perm,
-                    bool every_tile) {
+                    bool every_tile, const PgLayout &L) {
   hipStream_t s = p.stream;
   const TileGeom &tg = p.tg;
 
@@ -628,41 +917,83 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
   } else {
     const int nhi = 1 << tg.hi_bits, nlo = 1 << tg.lo_bits;
     ks = (int64_t)nhi * nlo;
-    // pass 1: high digit over [0, n)
-    uint32_t *t1 = static_cast<uint32_t *>(p.table1.ensure((size_t)nhi * g1 * 4));
     const DigitSel d1 = digit_sel(kg, tg.lo_bits, tg.hi_bits);
     const DigitSel d2 = digit_sel(kg, 0, tg.lo_bits);
     // with digits of at most 8 bits, pass 1 writes each record's low digit as
-    // a byte beside it, and pass 2 counts those bytes instead of reading the
-    // records
-    uint8_t *dig2 = tg.lo_bits <= 8 && tg.hi_bits <= 8 ? static_cast<uint8_t *>(p.dig2.ensure((size_t)n + 64))
-                                                        : nullptr;
-    ChunkMap cm1{n, 0, nullptr, nullptr, g1, 0, chunk};
-    {
-      Stage st(p, "sort_upsweep1");
-      hipLaunchKernelGGL((k_upsweep<NM>), dim3((unsigned)g1), dim3(SORT_THREADS), nhi * 4, s, raw, cm1, kg, d1, nhi,
-                         t1);
-      GENPK_LAUNCH_CHECK();
-    }
-    exclusive_scan_u32(t1, t1, (int64_t)nhi * g1, p.scan_tmp, s);
-    {
-      Stage st(p, "sort_downsweep1");
-      launch_downsweep<NM>(g1, s, raw, A, cm1, kg, d1, nhi, t1, perm1, dig2, d2, mrange);
-    }
-    // segments of pass 2
-    uint32_t *seg = static_cast<uint32_t *>(p.seginfo.ensure((size_t)(nhi + 1) * 3 * 4));
+    // a byte beside it (in its output slot), and pass 2 counts those bytes
+    // instead of reading the records
+    const int64_t slots = L.on ? L.slots : n;  // pass-1 output slots
+    uint8_t *dig2 = tg.lo_bits <= 8 && tg.hi_bits <= 8
+                        ? static_cast<uint8_t *>(p.dig2.ensure((size_t)slots + 64))
+                        : nullptr;
+    // segments of pass 2 (one per pass-1 digit): start (records, or list
+    // slots when paged), chunk count and first chunk, first record
+    uint32_t *seg = static_cast<uint32_t *>(p.seginfo.ensure((size_t)(nhi + 1) * 4 * 4));
     uint32_t *seg_start = seg, *nchunks = seg + (nhi + 1), *chunk_start = seg + 2 * (nhi + 1);
-    hipLaunchKernelGGL(k_seginfo, dim3((unsigned)ceil_div(nhi + 1, 256)), dim3(256), 0, s, t1, g1, nhi, n, chunk,
-                       seg_start, nchunks);
-    GENPK_LAUNCH_CHECK();
+    uint32_t *rec_start = seg_start;
+    int64_t g2 = 0;  // upper bound of pass-2 chunks
+    PgArgs pg;
+    if (L.on) {
+      // pass 1, paged: the positions are read once
+      GENPK_REQUIRE(chunk % PG_BLOCK == 0, "paged sort: pass-2 chunks must be whole blocks");
+      const int64_t nbk = L.G * L.Q, nt = (int64_t)nhi * L.G + 1;
+      uint32_t *pb = static_cast<uint32_t *>(p.pages.ensure((size_t)(4 * nbk + L.G + 4 * nt) * 4));
+      pg.hdr = pb;
+      pg.fill = pb + nbk;
+      pg.used = pb + 2 * nbk;
+      pg.Q = L.Q;
+      uint32_t *blist = pg.used + L.G;
+      uint32_t *bcnt = blist + nbk, *rcnt = bcnt + nt, *boff = rcnt + nt, *roff = boff + nt;
+      rec_start = seg + 3 * (nhi + 1);
+      ChunkMap cm1{n, 0, nullptr, nullptr, L.G, 0, L.chunk};
+      {
+        Stage st(p, "sort_downsweep1");
+        launch_downsweep<NM, 1>(L.G, s, raw, A, cm1, kg, d1, nhi, nullptr, perm1, dig2, d2, mrange, pg);
+      }
+      {
+        Stage st(p, "sort_pages");
+        hipLaunchKernelGGL(k_pg_count, dim3((unsigned)L.G), dim3(256), 0, s, pg, nhi, L.G, bcnt, rcnt);
+        GENPK_LAUNCH_CHECK();
+        exclusive_scan_u32(bcnt, boff, nt, p.scan_tmp, s);
+        exclusive_scan_u32(rcnt, roff, nt, p.scan_tmp, s);
+        hipLaunchKernelGGL(k_pg_list, dim3((unsigned)L.G), dim3(256), 0, s, pg, nhi, L.G, boff, blist);
+        GENPK_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_pg_seginfo, dim3((unsigned)ceil_div(nhi + 1, 256)), dim3(256), 0, s, boff, roff, nhi,
+                           L.G, chunk, seg_start, rec_start, nchunks);
+        GENPK_LAUNCH_CHECK();
+      }
+      pg.blist = blist;
+      g2 = ceil_div(nbk * PG_BLOCK, chunk) + nhi;
+    } else {
+      // pass 1: high digit over [0, n)
+      uint32_t *t1 = static_cast<uint32_t *>(p.table1.ensure((size_t)nhi * g1 * 4));
+      ChunkMap cm1{n, 0, nullptr, nullptr, g1, 0, chunk};
+      {
+        Stage st(p, "sort_upsweep1");
+        hipLaunchKernelGGL((k_upsweep<NM>), dim3((unsigned)g1), dim3(SORT_THREADS), nhi * 4, s, raw, cm1, kg, d1,
+                           nhi, t1);
+        GENPK_LAUNCH_CHECK();
+      }
+      exclusive_scan_u32(t1, t1, (int64_t)nhi * g1, p.scan_tmp, s);
+      {
+        Stage st(p, "sort_downsweep1");
+        launch_downsweep<NM>(g1, s, raw, A, cm1, kg, d1, nhi, t1, perm1, dig2, d2, mrange);
+      }
+      hipLaunchKernelGGL(k_seginfo, dim3((unsigned)ceil_div(nhi + 1, 256)), dim3(256), 0, s, t1, g1, nhi, n, chunk,
+                         seg_start, nchunks);
+      GENPK_LAUNCH_CHECK();
+      g2 = g1 + nhi;
+    }
     exclusive_scan_u32(nchunks, chunk_start, nhi + 1, p.scan_tmp, s);
-    const int64_t g2 = g1 + nhi;  // upper bound of pass-2 chunks
     uint32_t *t2 = static_cast<uint32_t *>(p.table2.ensure((size_t)g2 * nlo * 4));
     ChunkMap cm2{n, nhi, seg_start, chunk_start, 0, nlo, chunk};
     Src<NM> pa{A, nullptr, false};
     {
       Stage st(p, "sort_upsweep2");
-      if (dig2)
+      if (L.on)
+        hipLaunchKernelGGL(k_upsweep_pages, dim3((unsigned)g2), dim3(SORT_THREADS), nlo * 4, s, dig2, cm2, pg, nlo,
+                           t2);
+      else if (dig2)
         hipLaunchKernelGGL(k_upsweep_bytes, dim3((unsigned)g2), dim3(SORT_THREADS), nlo * 4, s, dig2, cm2, nlo, t2);
       else
         hipLaunchKernelGGL((k_upsweep<NM>), dim3((unsigned)g2), dim3(SORT_THREADS), nlo * 4, s, pa, cm2, kg, d2, nlo,
@@ -674,11 +1005,14 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     exclusive_scan_u32(t2, t2, g2 * nlo, p.scan_tmp, s);
     tile_start = static_cast<uint32_t *>(p.tile_start.ensure((size_t)(ks + 1) * 4));
     hipLaunchKernelGGL(k_tile_start_2pass, dim3((unsigned)ceil_div(ks + 1, 256)), dim3(256), 0, s, t2, chunk_start,
-                       seg_start, nhi, nlo, n, tile_start);
+                       rec_start, nhi, nlo, n, tile_start);
     GENPK_LAUNCH_CHECK();
     {
       Stage st(p, "sort_downsweep2");
-      launch_downsweep<NM>(g2, s, pa, B, cm2, kg, d2, nlo, t2, perm2);
+      if (L.on)
+        launch_downsweep<NM, 2>(g2, s, pa, B, cm2, kg, d2, nlo, t2, perm2, nullptr, DigitSel{0, 0, 0}, nullptr, pg);
+      else
+        launch_downsweep<NM>(g2, s, pa, B, cm2, kg, d2, nlo, t2, perm2);
     }
     sorted = B;
   }
@@ -736,17 +1070,19 @@ TileOrder sort_by_tile(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, 
                        bool every_tile) {
   GENPK_REQUIRE(n > 0 && n <= ((int64_t)1 << 31), "sort_by_tile: particle count out of range");
   const int rw = 3 + (masses ? 1 : 0);
+  const PgLayout L = pg_layout(p, n, rw);
   // with the permutations, both record buffers can also take the adjoint
-  // gather's output records (up to 4 floats)
+  // gather's output records (up to 4 floats); recA holds the pass-1 output
+  // (paged: L.slots record slots), perm2 is indexed by its slots
   const size_t rec_bytes = (size_t)n * (with_perm ? 4 : rw) * 4;
-  p.recA.ensure(rec_bytes);
+  p.recA.ensure(L.on ? std::max(rec_bytes, (size_t)L.slots * rw * 4) : rec_bytes);
   if (p.tg.passes == 2 || with_perm) p.recB.ensure(rec_bytes);
   if (with_perm) {
     p.permA.ensure((size_t)n * 4);
-    if (p.tg.passes == 2) p.permB.ensure((size_t)n * 4);
+    if (p.tg.passes == 2) p.permB.ensure((size_t)(L.on ? L.slots : n) * 4);
   }
-  if (masses) return sort_impl<1>(p, kg, n, pos, masses, with_perm, every_tile);
-  return sort_impl<0>(p, kg, n, pos, nullptr, with_perm, every_tile);
+  if (masses) return sort_impl<1>(p, kg, n, pos, masses, with_perm, every_tile, L);
+  return sort_impl<0>(p, kg, n, pos, nullptr, with_perm, every_tile, L);
 }
 
 }  // namespace genpk

@@ -1,0 +1,92 @@
+"""The paged first pass of the tile sort (2^29 or more particles without
+masses on a 1024^3 grid): pass 1 appends each digit's records to blocks of
+record slots, with no histogram pass, and pass 2 reads the blocks of each
+digit through a block list.  The paged path is checked against the plain
+two-pass path on the same particles, split in halves below the paging
+threshold, where the results do not depend on how the records were
+partitioned:
+
+* the FP64 density: each deposit's per-cell sums are exact integers in
+  fixed point (2^-36 per particle weight), and the two halves' sums add
+  exactly in FP64 below 2^17 particle weights per cell; the clumps' hottest
+  cells exceed that, so the bar is one rounding of the sum (2^-52
+  relative), far below one particle's weight;
+* the adjoint gather's gradients (per particle, from the same dL/drho),
+  through both the order a retaining deposit kept (perm1 into the paged
+  pool, perm2 out of it) and a gather that partitions the particles itself;
+* redeposit() from the kept paged order.
+
+The clustered 1024^3 parity test against the FP64 host oracle runs the
+paged path at 2^30 particles.
+"""
+import pytest
+
+import genpk_amd
+from genpk_amd import synth
+
+
+def announce(*a):
+    print("CASE", *a, flush=True)
+
+
+def _stages(plan):
+    return set(plan.timing_read())
+
+
+@pytest.mark.gpu
+def test_paged_sort_matches_plain_halves(gpu):
+    import torch
+    n, d, box = 1 << 29, 1024, 1000.0
+    h = n // 2
+    announce("paged sort vs plain halves", f"n=2^29 d={d} f64")
+    pos = synth.clumped_torch(n, box, d, n_clumps=20000, seed=11)
+    # the paged pass at n, the plain passes at n / 2
+    a = genpk_amd.Plan(d, "f64")
+    a.timing(True)
+    a.clear()
+    a.deposit(pos, box, retain=True)
+    torch.cuda.synchronize()
+    st = _stages(a)
+    assert "sort_pages" in st and "sort_upsweep1" not in st, st
+    ref = a.export_density().clone()
+    b = genpk_amd.Plan(d, "f64")
+    b.timing(True)
+    b.clear()
+    b.deposit(pos[:h], box)
+    b.deposit(pos[h:], box)
+    torch.cuda.synchronize()
+    st = _stages(b)
+    assert "sort_upsweep1" in st and "sort_pages" not in st, st
+    got = b.export_density()
+    diff = (got - ref).abs()
+    rel = (diff / ref.abs().clamp_min(1e-300)).max().item()
+    print(f"cells differing {(diff > 0).sum().item()}, max |diff| {diff.max().item():.3e}, "
+          f"max rel {rel:.3e}, max cell {ref.max().item():.6e}", flush=True)
+    assert diff[ref.abs() < 2.0 ** 17].max().item() == 0.0
+    assert (diff <= 2.0 ** -52 * ref.abs()).all()
+    del got, diff
+    del b
+    torch.cuda.empty_cache()
+    # redeposit from the kept paged order
+    a.clear()
+    a.redeposit()
+    assert torch.equal(a.export_density(), ref)
+    del ref
+    # gradients from one dL/drho: the kept order, a fresh paged partition,
+    # and plain halves
+    G = torch.randn(a.export_density().shape, dtype=torch.float64, device="cuda",
+                    generator=torch.Generator(device="cuda").manual_seed(5))
+    a.clear()
+    a.deposit(pos, box, retain=True)
+    assert a.keeps_order_of(pos, box)
+    a.import_density(G)
+    g_kept = a.adjoint_gather(pos, box, want_mass_grad=False)[0]
+    c = genpk_amd.Plan(d, "f64")
+    c.import_density(G)
+    g_fresh = c.adjoint_gather(pos, box, want_mass_grad=False)[0]
+    g_lo = c.adjoint_gather(pos[:h], box, want_mass_grad=False)[0]
+    g_hi = c.adjoint_gather(pos[h:], box, want_mass_grad=False)[0]
+    torch.cuda.synchronize()
+    assert torch.isfinite(g_kept).all()
+    assert torch.equal(g_kept, g_fresh)
+    assert torch.equal(g_kept[:h], g_lo) and torch.equal(g_kept[h:], g_hi)
