@@ -2,9 +2,12 @@
 // of one or two passes over the tile key (upsweep histogram in LDS ->
 // offset scan -> downsweep that ranks records with LDS atomics, stages them
 // in LDS by digit and writes contiguous runs), then the work items of the
-// tile kernels.  Global atomics are avoided: on this chip they run at
-// ~27 Gop/s.  The order of the records within one tile is not specified
-// (the tile kernels do not depend on it).
+// tile kernels.  Large two-pass sorts run their first pass paged: each
+// workgroup appends each digit's records to blocks of its own, so the
+// positions are read once (no histogram pass), and the second pass reads
+// every digit's blocks through a block list.  Global atomics are avoided:
+// on this chip they run at ~27 Gop/s.  The order of the records within one
+// tile is not specified (the tile kernels do not depend on it).
 #include "tilesort.h"
 
 #include <algorithm>
@@ -837,12 +840,16 @@ void launch_downsweep(int64_t grid, hipStream_t s, Src<NM> src, float *dst, cons
   GENPK_LAUNCH_CHECK();
 }
 
-// Whether and how a sort runs its first pass paged: records without masses,
-// two passes with digits of at most 8 bits, and so many records that the
-// partly filled blocks (at most one per pass-1 workgroup and digit) stay
-// below 1/8 of them (half of that on average).  Each of the G = CU-count workgroups takes `chunk`
-// records (whole sub-chunks) and Q = chunk / PG_BLOCK + nhi blocks: every
-// digit's blocks but its last are full.
+// Whether and how a sort runs its first pass paged: two passes with digits
+// of at most 8 bits, and so many records that the partly filled blocks (at
+// most one per pass-1 workgroup and digit) leave fewer empty slots than 1/4
+// of the records (half that on average).  G = twice the CU count workgroups
+// (two rounds of one per CU even out their finishing times: 7.8 -> 7.5 ms
+// at 2^30 records) take `chunk` records each (whole sub-chunks) and Q =
+// chunk / PG_BLOCK + nhi blocks: every digit's blocks but its last are
+// full.  With masses (16-byte records) the paged downsweeps keep a few
+// registers in scratch and are still faster than the histogram pass they
+// replace (37.5 -> 36.3 ms per retaining forward at 2^30).
 struct PgLayout {
   bool on = false;
   int64_t G = 0, chunk = 0, Q = 0;
@@ -852,15 +859,13 @@ struct PgLayout {
 PgLayout pg_layout(const Plan &p, int64_t n, int rw) {
   PgLayout L;
   const TileGeom &tg = p.tg;
-  // records of positions only: with masses (16-byte records) the paged
-  // downsweeps would spill registers
-  if (rw != 3 || tg.passes != 2 || tg.hi_bits > 8 || tg.lo_bits > 8) return L;
+  if (tg.passes != 2 || tg.hi_bits > 8 || tg.lo_bits > 8) return L;
   const int64_t nhi = (int64_t)1 << tg.hi_bits, cus = num_cus();
-  if (cus * nhi * (int64_t)PG_BLOCK > n / 8) return L;
+  if (2 * cus * nhi * (int64_t)PG_BLOCK > n / 4) return L;
   for (int mode : {1, 2})
     if (downsweep_shape((int)nhi, rw, mode).pl != 8) return L;
   constexpr int64_t SUBN = (int64_t)SORT_THREADS * 8;
-  L.chunk = ceil_div(ceil_div(n, cus), SUBN) * SUBN;
+  L.chunk = ceil_div(ceil_div(n, 2 * cus), SUBN) * SUBN;
   L.G = ceil_div(n, L.chunk);
   L.Q = L.chunk / PG_BLOCK + nhi;
   L.slots = L.G * L.Q * PG_BLOCK;
