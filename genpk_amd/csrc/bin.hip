@@ -144,7 +144,7 @@ struct BinTrack {
 // binning, each thread pairing stage inputs j and j + n/2 into modes j and
 // j + n/2 in registers; the two buffers then swap roles between blocks so
 // that the next first stage never writes the buffer being binned.
-template <typename T, int NF, bool REG1, bool R2L = false>
+template <typename T, int NF, bool REG1, bool R2L = false, int CT = 0>
 __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__restrict__ fa,
                                                             typename Cx<T>::C *__restrict__ fb, BinGeom g, int B,
                                                             int nkb, RadixPlan rp,
@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
   extern __shared__ unsigned char lds_raw[];
   double *bins = reinterpret_cast<double *>(lds_raw);
   C *buf = reinterpret_cast<C *>(bins + ((NS * g.nrbins + 1) & ~1));  // [NF][n][B], 16-byte aligned
-  const int n = rp.n;
+  const int n = CT ? CT : rp.n;
   const int lgb = ilog2(B);
   // pp (one field): a second buffer lets the FFT stages alternate (one barrier each)
   C *buf2 = buf + (size_t)NF * n * B;
@@ -209,7 +209,10 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
     if constexpr (REG1) {
       fft_first_regs<MAXE, -1>(va, buf2, B, lgb, B, 1);
       __syncthreads();
-      res = lds_fft_pp<-1, true, MAXE>(buf2, buf, rp, B, lgb, B, 1, stw, 1, R2L ? rp.nst - 1 : -1);
+      if constexpr (CT == 1024 && R2L && MAXE == 8)
+        res = lds_fft_ct<1024, 8, -1, true, MAXE, true, C, false, 8, 8>(buf2, buf, B, lgb, stw);
+      else
+        res = lds_fft_pp<-1, true, MAXE>(buf2, buf, rp, B, lgb, B, 1, stw, 1, R2L ? rp.nst - 1 : -1);
     } else {
       __syncthreads();  // the previous block's binning is done with buf
 #pragma unroll
@@ -559,17 +562,17 @@ static bool fused_reg1(const Plan &a, int B, bool pp) {
 
 // Persistent grid of the fused kernel: as many workgroups as stay resident
 // (occupancy of this instance x CUs), at most one per column block.
-template <typename T, int NF, bool REG1, bool R2L = false>
+template <typename T, int NF, bool REG1, bool R2L = false, int CT = 0>
 static int fused_grid(const Plan &a, int B, int nrbins, bool pp) {
   int per_cu = 0;
   GENPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
-                                                          reinterpret_cast<const void *>(&k_xpass_bin<T, NF, REG1, R2L>),
+                                                          reinterpret_cast<const void *>(&k_xpass_bin<T, NF, REG1, R2L, CT>),
                                                           BIN_THREADS, fused_lds<T, NF>(a, B, nrbins, pp)));
   const int64_t nblk = a.d * ceil_div(a.hz, B);
   return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)std::max(per_cu, 1) * num_cus(), nblk));
 }
 
-template <typename T, int NF, bool REG1, bool R2L = false>
+template <typename T, int NF, bool REG1, bool R2L = false, int CT = 0>
 static void launch_fused(Plan &a, Plan *b, const BinTables &bt, int B, bool keep, bool pp, double *part, int G) {
   using C = typename Cx<T>::C;
   constexpr int NS = NF == 1 ? 1 : 3;
@@ -581,7 +584,7 @@ static void launch_fused(Plan &a, Plan *b, const BinTables &bt, int B, bool keep
   BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), bt.thr.as<int64_t>(), a.wtab.as<float>(), bt.lq};
   const size_t lds = fused_lds<T, NF>(a, B, bt.nrbins, pp);
   Stage st(a, "xpass_bin");
-  hipLaunchKernelGGL((k_xpass_bin<T, NF, REG1, R2L>), dim3(G), dim3(BIN_THREADS), lds, a.stream, a.field.as<C>(),
+  hipLaunchKernelGGL((k_xpass_bin<T, NF, REG1, R2L, CT>), dim3(G), dim3(BIN_THREADS), lds, a.stream, a.field.as<C>(),
                      b ? b->field.as<C>() : (C *)nullptr, g, B, nkb, rp, tw, keep ? 1 : 0, pp ? 1 : 0, part);
   GENPK_LAUNCH_CHECK();
 }
@@ -599,12 +602,31 @@ static void launch_stored(Plan &a, Plan *b, const BinTables &bt, double *part, i
 
 // Launch shape of the fused x pass + bin: column block B, ping-pong buffers
 // (pp), the register first stage (reg1), the radix-2 last stage in the
-// binning (r2l), and the persistent grid G.
+// binning (r2l), compile-time stages for the n = 1024 plan 8, 8, 8, 2 (ct,
+// with r2l), and the persistent grid G.
 struct FusedShape {
   int B = 0;
-  bool pp = false, reg1 = false, r2l = false;
+  bool pp = false, reg1 = false, r2l = false, ct = false;
   int G = 0;
 };
+
+template <typename T>
+static int fused_grid_of(const Plan &a, bool cross, const FusedShape &s, int nrbins) {
+  if (cross) return fused_grid<T, 2, false>(a, s.B, nrbins, false);
+  if (s.r2l && s.ct) return fused_grid<T, 1, true, true, 1024>(a, s.B, nrbins, s.pp);
+  if (s.r2l) return fused_grid<T, 1, true, true>(a, s.B, nrbins, s.pp);
+  if (s.reg1) return fused_grid<T, 1, true>(a, s.B, nrbins, s.pp);
+  return fused_grid<T, 1, false>(a, s.B, nrbins, s.pp);
+}
+
+template <typename T>
+static void launch_fused_of(Plan &a, Plan *b, const BinTables &bt, const FusedShape &s, double *part) {
+  if (b) launch_fused<T, 2, false>(a, b, bt, s.B, false, false, part, s.G);
+  else if (s.r2l && s.ct) launch_fused<T, 1, true, true, 1024>(a, b, bt, s.B, false, s.pp, part, s.G);
+  else if (s.r2l) launch_fused<T, 1, true, true>(a, b, bt, s.B, false, s.pp, part, s.G);
+  else if (s.reg1) launch_fused<T, 1, true>(a, b, bt, s.B, false, s.pp, part, s.G);
+  else launch_fused<T, 1, false>(a, b, bt, s.B, false, s.pp, part, s.G);
+}
 
 static FusedShape fused_shape(const Plan &a, const Plan *b, int nrbins) {
   const int nf = b ? 2 : 1, ns = b ? 3 : 1;
@@ -619,14 +641,11 @@ static FusedShape fused_shape(const Plan &a, const Plan *b, int nrbins) {
   // FP32 with a last stage of radix 2: that stage runs in the binning
   const RadixPlan &xrp = a.rp_d;
   s.r2l = s.reg1 && a.prec == 4 && xrp.st[xrp.nst - 1].R == 2;
-  const bool pp = s.pp;
-  if (a.prec == 4)
-    s.G = b ? fused_grid<float, 2, false>(a, B, nrbins, false)
-            : s.r2l ? fused_grid<float, 1, true, true>(a, B, nrbins, pp)
-                    : s.reg1 ? fused_grid<float, 1, true>(a, B, nrbins, pp) : fused_grid<float, 1, false>(a, B, nrbins, pp);
-  else
-    s.G = b ? fused_grid<double, 2, false>(a, B, nrbins, false)
-            : s.reg1 ? fused_grid<double, 1, true>(a, B, nrbins, pp) : fused_grid<double, 1, false>(a, B, nrbins, pp);
+  // compile-time stages where they pay: the FP32 register-first instance at
+  // n = 1024 (the others would spill registers)
+  s.ct = s.r2l && is_plan_8882(xrp);
+  const bool cross = b != nullptr;
+  s.G = a.prec == 4 ? fused_grid_of<float>(a, cross, s, nrbins) : fused_grid_of<double>(a, cross, s, nrbins);
   return s;
 }
 
@@ -656,20 +675,12 @@ void bin_spectra(Plan &a, Plan *b, int nrbins, bool fused_xpass, double *pow_aa,
     GENPK_REQUIRE(a.fstate == FieldState::YDone && (!b || b->fstate == FieldState::YDone),
                   "fused binning needs fields after the z and y passes");
     const FusedShape fs = fused_shape(a, b, nrbins);
-    const int B = fs.B;
-    const bool pp = fs.pp, reg1 = fs.reg1, r2l = fs.r2l;
     G = fs.G;
     part = partials((int64_t)G * ns * nrbins);
-    if (a.prec == 4) {
-      if (b) launch_fused<float, 2, false>(a, b, bt, B, false, false, part, G);
-      else if (r2l) launch_fused<float, 1, true, true>(a, b, bt, B, false, pp, part, G);
-      else if (reg1) launch_fused<float, 1, true>(a, b, bt, B, false, pp, part, G);
-      else launch_fused<float, 1, false>(a, b, bt, B, false, pp, part, G);
-    } else {
-      if (b) launch_fused<double, 2, false>(a, b, bt, B, false, false, part, G);
-      else if (reg1) launch_fused<double, 1, true>(a, b, bt, B, false, pp, part, G);
-      else launch_fused<double, 1, false>(a, b, bt, B, false, pp, part, G);
-    }
+    if (a.prec == 4)
+      launch_fused_of<float>(a, b, bt, fs, part);
+    else
+      launch_fused_of<double>(a, b, bt, fs, part);
     // the field keeps the forward state after the z and y passes (an adjoint can start from it)
     a.fstate = FieldState::YDone;
     if (b) b->fstate = FieldState::YDone;

@@ -39,8 +39,9 @@ template <> struct FftCfg<double> { static constexpr int ELEMS = 4096; static co
 // BFLY (FP32, from the tile store, M == FFT_THREADS, a first stage of radix
 // 8): each thread loads the eight inputs of up to two butterflies of the
 // first stage straight from the tile store and runs that stage from
-// registers.
-template <typename T, bool FROM_TILES, bool BFLY = false>
+// registers.  CT = 512: the plan is M = 512 = 8 * 8 * 8 (d = 1024), run with
+// compile-time stages (fft_stage_ct); 0: any plan.
+template <typename T, bool FROM_TILES, bool BFLY = false, int CT = 0>
 __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(1, 2))) k_r2c_z(T *__restrict__ field, const T *__restrict__ tiles,
                                                       FoldGeom fg, FastDiv fdd, int64_t rowlen, int64_t nrows, int L,
                                                       FastDiv fdM, FastDiv fdM1, RadixPlan rp,
@@ -50,7 +51,7 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
   constexpr int MAXE = FftCfg<T>::MAXE;
   extern __shared__ unsigned char lds_raw[];
   C *buf = reinterpret_cast<C *>(lds_raw);
-  const int M = rp.n;
+  const int M = CT ? CT : rp.n;
   const int64_t row0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * L;
   const int nl = (int)min((int64_t)L, nrows - row0);
   const int tot = L * M;
@@ -103,8 +104,11 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
     for (int i = threadIdx.x; i < M; i += FFT_THREADS) stw[i] = twM[i];
     for (int i = threadIdx.x; i <= M; i += FFT_THREADS) sxw[i] = twd[i];
     __syncthreads();
-    for (int st = 1; st < rp.nst; st++)
-      fft_stage_r<-1, false, MAXE, false, C, true>(rp.st[st].R, buf, buf, L, 0, 1, M, rp.st[st], stw);
+    if constexpr (CT == 512)
+      lds_fft_ct<512, 8, -1, false, MAXE, false, C, true, 8, 8>(buf, buf, L, 0, stw);
+    else
+      for (int st = 1; st < rp.nst; st++)
+        fft_stage_r<-1, false, MAXE, false, C, true>(rp.st[st].R, buf, buf, L, 0, 1, M, rp.st[st], stw);
   } else {
   C v[MAXE];
   T zh[FROM_TILES ? MAXE : 1];
@@ -172,7 +176,10 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
   for (int i = threadIdx.x; i < M; i += FFT_THREADS) stw[i] = twM[i];
   for (int i = threadIdx.x; i <= M; i += FFT_THREADS) sxw[i] = twd[i];
   __syncthreads();
-  lds_fft<-1, false, MAXE, C, true>(buf, rp, L, 0, 1, M, stw);
+  if constexpr (CT == 512)
+    lds_fft_ct<512, 1, -1, false, MAXE, false, C, true, 8, 8, 8>(buf, buf, L, 0, stw);
+  else
+    lds_fft<-1, false, MAXE, C, true>(buf, rp, L, 0, 1, M, stw);
   }
   // X_k = (Z_k + conj Z_{M-k})/2 - i/2 * w^k (Z_k - conj Z_{M-k}),  w = exp(-2 pi i / d)
   auto split = [&](C zk, C zc, C w) {
@@ -316,8 +323,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_c2r_z(T *__restrict__ field, in
 }
 
 // c2c along y (axis = 1) or x (axis = 0) over column blocks of B = 1 << lgb
-// consecutive kz values.
-template <typename T, int SIGN>
+// consecutive kz values.  CT = 1024: the plan is n = 1024 = 8 * 8 * 8 * 2,
+// run with compile-time stages; 0: any plan.
+template <typename T, int SIGN, int CT = 0>
 __global__ void __launch_bounds__(COL_THREADS) k_cols(typename Cx<T>::C *__restrict__ f, int64_t d, int64_t hz,
                                                      int lgb, int nkb, int axis, RadixPlan rp,
                                                      const typename Cx<T>::C *__restrict__ tw) {
@@ -325,7 +333,7 @@ __global__ void __launch_bounds__(COL_THREADS) k_cols(typename Cx<T>::C *__restr
   constexpr int MAXE = FftCfg<T>::COLE;
   extern __shared__ unsigned char lds_raw[];
   C *buf = reinterpret_cast<C *>(lds_raw);  // [n][B]
-  const int n = rp.n;
+  const int n = CT ? CT : rp.n;
   const int B = 1 << lgb;
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   const int64_t outer = blk / nkb;
@@ -350,7 +358,10 @@ __global__ void __launch_bounds__(COL_THREADS) k_cols(typename Cx<T>::C *__restr
   C *stw = buf + tot;  // twiddles staged in LDS
   for (int i = threadIdx.x; i < n; i += COL_THREADS) stw[i] = tw[i];
   __syncthreads();
-  lds_fft<SIGN, true, MAXE>(buf, rp, B, lgb, B, 1, stw);
+  if constexpr (CT == 1024)
+    lds_fft_ct<1024, 1, SIGN, true, MAXE, false, C, false, 8, 8, 8, 2>(buf, buf, B, lgb, stw);
+  else
+    lds_fft<SIGN, true, MAXE>(buf, rp, B, lgb, B, 1, stw);
 #pragma unroll
   for (int q = 0; q < MAXE; q++) {
     const int i = threadIdx.x + q * COL_THREADS;
@@ -365,8 +376,9 @@ __global__ void __launch_bounds__(COL_THREADS) k_cols(typename Cx<T>::C *__restr
 // addresses as k_cols' element-wise load), so the first stage runs from
 // registers and the load's LDS round trip and two barriers are saved.
 // L2 (last radix 2): the last stage's butterflies (rows k and k + n/2 of a
-// column, twiddle w^k) write their outputs straight to global memory.
-template <typename T, int SIGN, bool L2>
+// column, twiddle w^k) write their outputs straight to global memory.  CT as
+// k_cols.
+template <typename T, int SIGN, bool L2, int CT = 0>
 __global__ void __launch_bounds__(COL_THREADS) k_cols_r1(typename Cx<T>::C *__restrict__ f, int64_t d, int64_t hz,
                                                         int lgb, int nkb, int axis, RadixPlan rp,
                                                         const typename Cx<T>::C *__restrict__ tw) {
@@ -374,7 +386,7 @@ __global__ void __launch_bounds__(COL_THREADS) k_cols_r1(typename Cx<T>::C *__re
   constexpr int R = FftCfg<T>::COLE;
   extern __shared__ unsigned char lds_raw[];
   C *buf = reinterpret_cast<C *>(lds_raw);  // [n][B]
-  const int n = rp.n;
+  const int n = CT ? CT : rp.n;
   const int B = 1 << lgb;
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   const int64_t outer = blk / nkb;
@@ -390,8 +402,11 @@ __global__ void __launch_bounds__(COL_THREADS) k_cols_r1(typename Cx<T>::C *__re
   for (int i = threadIdx.x; i < n; i += COL_THREADS) stw[i] = tw[i];
   fft_first_regs<R, SIGN>(v, buf, B, lgb, B, 1);
   __syncthreads();
-  for (int s = 1; s < rp.nst - (L2 ? 1 : 0); s++)
-    fft_stage_r<SIGN, true, R, false, C, false>(rp.st[s].R, buf, buf, B, lgb, B, 1, rp.st[s], stw);
+  if constexpr (CT == 1024 && L2 && R == 8)
+    lds_fft_ct<1024, 8, SIGN, true, R, false, C, false, 8, 8>(buf, buf, B, lgb, stw);
+  else
+    for (int s = 1; s < rp.nst - (L2 ? 1 : 0); s++)
+      fft_stage_r<SIGN, true, R, false, C, false>(rp.st[s].R, buf, buf, B, lgb, B, 1, rp.st[s], stw);
   if constexpr (L2) {
     const int h = n / 2;
 #pragma unroll
@@ -476,9 +491,21 @@ static void r2c_z_impl(Plan &p, bool from_tiles) {
   const size_t lds = ((size_t)lds_len<true>((int64_t)L * M) + 2 * M + 1) * sizeof(C);
   const bool bfly = sizeof(T) == 4 && M == FFT_THREADS && rp.st[0].R == 8 && rp.nst >= 2 &&
                     (int64_t)L * (M / 8) <= 2 * FFT_THREADS;
-  if (from_tiles && bfly) {
+  // the d = 1024 plan of radices 8, 8, 8 runs compile-time stages
+  const bool ct512 = M == 512 && rp.nst == 3 && rp.st[0].R == 8 && rp.st[1].R == 8 && rp.st[2].R == 8;
+  if (from_tiles && bfly && ct512) {
+    Stage st(p, "fold_fft_z");
+    hipLaunchKernelGGL((k_r2c_z<T, true, true, 512>), grid, dim3(FFT_THREADS), lds, p.stream, p.field.as<T>(),
+                       p.tiles.as<const T>(), fg, FastDiv((uint32_t)p.d), p.rowlen, nrows, L, FastDiv((uint32_t)M),
+                       FastDiv((uint32_t)(M + 1)), rp, twM, twd);
+  } else if (from_tiles && bfly) {
     Stage st(p, "fold_fft_z");
     hipLaunchKernelGGL((k_r2c_z<T, true, true>), grid, dim3(FFT_THREADS), lds, p.stream, p.field.as<T>(),
+                       p.tiles.as<const T>(), fg, FastDiv((uint32_t)p.d), p.rowlen, nrows, L, FastDiv((uint32_t)M),
+                       FastDiv((uint32_t)(M + 1)), rp, twM, twd);
+  } else if (from_tiles && ct512) {
+    Stage st(p, "fold_fft_z");
+    hipLaunchKernelGGL((k_r2c_z<T, true, false, 512>), grid, dim3(FFT_THREADS), lds, p.stream, p.field.as<T>(),
                        p.tiles.as<const T>(), fg, FastDiv((uint32_t)p.d), p.rowlen, nrows, L, FastDiv((uint32_t)M),
                        FastDiv((uint32_t)(M + 1)), rp, twM, twd);
   } else if (from_tiles) {
@@ -521,13 +548,20 @@ static void cols_impl(Plan &p, int axis) {
   const size_t lds = (size_t)p.d * (B + 1) * sizeof(C);
   const int64_t nblk = p.d * nkb;
   Stage st(p, axis == 1 ? (SIGN < 0 ? "fft_y" : "ifft_y") : (SIGN < 0 ? "fft_x" : "ifft_x"));
+  const bool ct1024 = is_plan_8882(rp);
   if (rp.st[0].R == FftCfg<T>::COLE && p.d * B == (int64_t)COL_THREADS * FftCfg<T>::COLE) {
-    if (rp.st[rp.nst - 1].R == 2)
+    if (rp.st[rp.nst - 1].R == 2 && ct1024)
+      hipLaunchKernelGGL((k_cols_r1<T, SIGN, true, 1024>), dim3((unsigned)nblk), dim3(COL_THREADS), lds, p.stream,
+                         p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, axis, rp, tw);
+    else if (rp.st[rp.nst - 1].R == 2)
       hipLaunchKernelGGL((k_cols_r1<T, SIGN, true>), dim3((unsigned)nblk), dim3(COL_THREADS), lds, p.stream,
                          p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, axis, rp, tw);
     else
       hipLaunchKernelGGL((k_cols_r1<T, SIGN, false>), dim3((unsigned)nblk), dim3(COL_THREADS), lds, p.stream,
                          p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, axis, rp, tw);
+  } else if (ct1024) {
+    hipLaunchKernelGGL((k_cols<T, SIGN, 1024>), dim3((unsigned)nblk), dim3(COL_THREADS), lds, p.stream,
+                       p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, axis, rp, tw);
   } else {
     hipLaunchKernelGGL((k_cols<T, SIGN>), dim3((unsigned)nblk), dim3(COL_THREADS), lds, p.stream, p.field.as<C>(), p.d,
                        p.hz, ilog2(B), nkb, axis, rp, tw);

@@ -191,6 +191,12 @@ inline RadixPlan make_radix_plan(int n, int variant = 0) {
   return p;
 }
 
+// The column-pass plan that runs compile-time stages (fft_stage_ct): n =
+// 1024 as radices 8, 8, 8, 2 (d = 1024, radix ordering 0).
+inline bool is_plan_8882(const RadixPlan &p) {
+  return p.n == 1024 && p.nst == 4 && p.st[0].R == 8 && p.st[1].R == 8 && p.st[2].R == 8 && p.st[3].R == 2;
+}
+
 inline bool fft_length_ok(int64_t n) {
   if (n < 1 || n > (1 << 20)) return false;
   return make_radix_plan((int)n).nst >= 0;
@@ -303,6 +309,82 @@ __device__ __forceinline__ C *lds_fft_pp(C *a, C *b, const RadixPlan &rp, int nb
     b = t;
   }
   return a;
+}
+
+// fft_stage with the radix, the length N and the span NS (product of the
+// previous radices) known at compile time, all powers of two: the butterfly
+// and twiddle indices are shifts and masks, and in the row layout (element
+// e of transform b at b * N + e) the R reads of a butterfly sit at
+// compile-time offsets from one address.  Same layouts and barriers as
+// fft_stage (se = nbatch, sb = 1 when BFAST; se = 1, sb = N otherwise).
+template <int R, int N, int NS, int SIGN, bool BFAST, int MAXE, bool PP, typename C, bool PAD>
+__device__ __forceinline__ void fft_stage_ct(const C *in, C *out, int nbatch, int lgb, const C *__restrict__ tw) {
+  static_assert((N & (N - 1)) == 0 && (NS & (NS - 1)) == 0 && N % (NS * R) == 0, "powers of two");
+  constexpr uint32_t NQ = N / R, TWS = N / (NS * R);
+  constexpr int MAXB = (MAXE + R - 1) / R;
+  const uint32_t nbf = (uint32_t)nbatch * NQ;
+  const uint32_t se = BFAST ? (uint32_t)nbatch : 1u;
+  C v[MAXB][R];
+  uint32_t dsto[MAXB];
+#pragma unroll
+  for (int i = 0; i < MAXB; i++) {
+    const uint32_t u = threadIdx.x + i * blockDim.x;
+    if (u < nbf) {
+      uint32_t b, j;
+      if (BFAST) {
+        b = u & (uint32_t)(nbatch - 1);
+        j = u >> lgb;
+      } else {
+        b = u / NQ;
+        j = u % NQ;
+      }
+      const uint32_t jq = j / NS, k = j % NS;
+      const uint32_t i0 = BFAST ? b + j * se : b * N + j;
+      auto src = [&](int r) -> C {
+        if constexpr (PAD) return in[lds_ix<true>((int)(i0 + r * NQ * se))];
+        else return in[i0 + r * NQ * se];
+      };
+      v[i][0] = src(0);
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        if constexpr (NS == 1) {
+          v[i][r] = src(r);
+        } else {
+          C t = tw[r * k * TWS];
+          if (SIGN > 0) t.y = -t.y;
+          v[i][r] = cmul(src(r), t);
+        }
+      }
+      dsto[i] = (BFAST ? b : b * N) + (jq * NS * R + k) * se;
+    }
+  }
+  if (!PP) __syncthreads();
+#pragma unroll
+  for (int i = 0; i < MAXB; i++) {
+    const uint32_t u = threadIdx.x + i * blockDim.x;
+    if (u < nbf) {
+      Dft<R, SIGN, C>::run(v[i]);
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        if constexpr (PAD) out[lds_ix<true>((int)(dsto[i] + r * NS * se))] = v[i][r];
+        else out[dsto[i] + r * NS * se] = v[i][r];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Stages R, RS... of a length-N transform starting at span NS (fft_stage_ct
+// each), alternating between buffers a and b when PP, in place otherwise;
+// returns the buffer that holds the result.
+template <int N, int NS, int SIGN, bool BFAST, int MAXE, bool PP, typename C, bool PAD, int R, int... RS>
+__device__ __forceinline__ C *lds_fft_ct(C *a, C *b, int nbatch, int lgb, const C *__restrict__ tw) {
+  C *dst = PP ? b : a;
+  fft_stage_ct<R, N, NS, SIGN, BFAST, MAXE, PP, C, PAD>(a, dst, nbatch, lgb, tw);
+  if constexpr (sizeof...(RS) > 0)
+    return lds_fft_ct<N, NS * R, SIGN, BFAST, MAXE, PP, C, PAD, RS...>(dst, PP ? a : b, nbatch, lgb, tw);
+  else
+    return dst;
 }
 
 // First stage (every twiddle 1) of a radix-R transform whose R inputs this
