@@ -162,11 +162,11 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
   C *buf2 = buf + (size_t)NF * n * B;
   C *stw = buf2 + (pp ? (size_t)n * B : 0);  // twiddles staged in LDS
   float *wt = reinterpret_cast<float *>(stw + n);  // deconvolution weights [d]
-  int *thr32 = reinterpret_cast<int *>(wt + g.d);  // bin thresholds [nrbins + 1]
+  int *thr32 = reinterpret_cast<int *>(wt + g.d);  // bin thresholds [nrbins]
   for (int i = threadIdx.x; i < n; i += BIN_THREADS) stw[i] = tw[i];
   for (int i = threadIdx.x; i < NS * g.nrbins; i += BIN_THREADS) bins[i] = 0;
   for (int i = threadIdx.x; i < g.d; i += BIN_THREADS) wt[i] = g.wtab[i];
-  for (int i = threadIdx.x; i <= g.nrbins; i += BIN_THREADS) thr32[i] = (int)g.thr[i];
+  for (int i = threadIdx.x; i < g.nrbins; i += BIN_THREADS) thr32[i] = (int)g.thr[i];
   const int64_t d = g.d, hz = g.hz;
   const int64_t estride = d * hz;
   const int64_t nblk = d * nkb;
@@ -538,7 +538,7 @@ static int fused_block(int prec, int64_t d, int64_t hz, int nf, int ns, int nrbi
   int B = col_block(prec, d, hz);
   while (B >= 1) {
     const int64_t lds = ((int64_t)nf * B + 1) * d * csize + (int64_t)((ns * nrbins + 1) & ~1) * 8 + d * 4 +
-                        (nrbins + 1) * 4;
+                        nrbins * 4;
     if (lds <= LDS_LIMIT && d * B <= elems && BIN_THREADS % B == 0) return B;
     B /= 2;
   }
@@ -550,7 +550,7 @@ static size_t fused_lds(const Plan &a, int B, int nrbins, bool pp) {
   using C = typename Cx<T>::C;
   constexpr int NS = NF == 1 ? 1 : 3;
   return ((size_t)NF * B + (pp ? B : 0) + 1) * a.d * sizeof(C) + (size_t)((NS * nrbins + 1) & ~1) * 8 +
-         (size_t)a.d * 4 + (size_t)(nrbins + 1) * 4;
+         (size_t)a.d * 4 + (size_t)nrbins * 4;
 }
 
 // Whether the REG1 instance applies (one field with ping-pong buffers).
