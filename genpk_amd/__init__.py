@@ -66,13 +66,17 @@ class Plan:
             check(lib().genpk_plan_attach_field(self._h, ctypes.c_void_p(self.grid.data_ptr())))
 
     # -- tuning options -------------------------------------------------
-    def set_options(self, tiles=(16, 32, 32), fft_variant=0):
+    def set_options(self, tiles=(16, 32, 32), fft_variant=None):
         """Deposit tile edges (powers of two, clamped to dims) and the FFT
         radix ordering (0: radix 8 first, 1: leftover radices first, 2: radix
-        4 first).  Drops the plan's deposits."""
+        4 first, 3: one radix 4 first in the y and x passes; None: the
+        precision's default, 3 for f64 and 0 for f32).  Drops the plan's
+        deposits."""
         self._sync_stream()
         self._kept = None
         self.generation += 1
+        if fft_variant is None:
+            fft_variant = 3 if self.prec == FLOAT64 else 0
         tx, ty, tz = (int(t) for t in tiles)
         check(lib().genpk_plan_set_options(self._h, tx, ty, tz, int(fft_variant)))
         return self

@@ -175,7 +175,7 @@ def _time_batch(b, torch, reps):
 
 # deposit tile shapes tried per grid size (x, y, z edges; clamped to dims)
 TILE_CANDIDATES = ((16, 32, 32), (8, 32, 32), (16, 16, 32), (8, 16, 32))
-FFT_VARIANTS = (0, 1, 2)
+FFT_VARIANTS = (0, 1, 2, 3)
 
 
 def _plan_key(dims, precision, torch):
@@ -211,7 +211,8 @@ def tune_plan_options(add_args, tiles=TILE_CANDIDATES, fft_variants=FFT_VARIANTS
         seen, timings, best = set(), {}, None
         plan = Plan(d, prec)
         for v in fft_variants:
-            rad = (tuple(radix_plan(d, v)), tuple(radix_plan(d // 2, v)))
+            # the z passes (d / 2) run ordering 0 under variant 3
+            rad = (tuple(radix_plan(d, v)), tuple(radix_plan(d // 2, 0 if v == 3 else v)))
             if rad in seen:
                 continue
             seen.add(rad)

@@ -35,7 +35,7 @@ Plan::Plan(int64_t dims, int precision, hipStream_t s) {
   rowlen = 2 * hz;
   field_elems = (size_t)d * d * rowlen;
   plan_set_tiles(*this, 16, 32, 32);
-  plan_set_fft_variant(*this, 0);
+  plan_set_fft_variant(*this, default_fft_variant(prec));
   field.ensure(field_elems * prec);
   const bool fftable = (d % 2 == 0) && fft_length_ok(d) && fft_length_ok(d / 2) && d >= 2;
   if (fftable) fft_tables_init(*this);
@@ -89,11 +89,13 @@ void plan_set_tiles(Plan &p, int tx, int ty, int tz) {
   p.order.valid = false;
 }
 
+// Variant 3 takes one radix 4 first in the column passes (n = d) only; the
+// z passes (n = d/2) keep ordering 0.
 void plan_set_fft_variant(Plan &p, int variant) {
   GENPK_REQUIRE(variant >= 0 && variant < FFT_VARIANTS, "unknown FFT radix variant");
   p.fft_variant = variant;
   p.rp_d = make_radix_plan((int)p.d, variant);
-  p.rp_h = make_radix_plan((int)std::max<int64_t>(p.d / 2, 1), variant);
+  p.rp_h = make_radix_plan((int)std::max<int64_t>(p.d / 2, 1), variant == 3 ? 0 : variant);
 }
 
 bool plan_fft_ok(const Plan &p) {

@@ -150,13 +150,19 @@ struct RadixPlan {
 // Radix orderings (variant): 0 = radix 8 first, then 4, 2, 3, 5 (the
 // fewest stages; the leftover small radices last); 1 = the same radices
 // with the leftover powers of two, 3s and 5s first and the 8s last; 2 =
-// radix 4 first (no 8), then 2, 3, 5 (more stages, smaller butterflies).
-constexpr int FFT_VARIANTS = 3;
+// radix 4 first (no 8), then 2, 3, 5 (more stages, smaller butterflies);
+// 3 = one radix 4 first, then as 0 (1024 = 4 8 8 4: a first stage that the
+// FP64 column kernels, four values per thread, run from registers; plans
+// use it for the column passes only).  Plans start with 3 in FP64 (at 1024^3
+// the y and fused x passes take 3.92 + 4.67 -> 3.83 + 4.15 ms) and 0 in FP32.
+constexpr int FFT_VARIANTS = 4;
+inline int default_fft_variant(int prec) { return prec == 8 ? 3 : 0; }
 
 inline RadixPlan make_radix_plan(int n, int variant = 0) {
   RadixPlan p;
   p.n = n;
   int r[32], nr = 0, m = n;
+  if (variant == 3 && m % 4 == 0) { r[nr++] = 4; m /= 4; }
   if (variant != 2)
     while (m % 8 == 0) { r[nr++] = 8; m /= 8; }
   while (m % 4 == 0) { r[nr++] = 4; m /= 4; }

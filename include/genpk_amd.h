@@ -59,14 +59,17 @@ int genpk_plan_attach_field(genpk_plan *plan, void *ptr);
  *     halo'd tile of 64-bit accumulators, (tile_x+1)(tile_y+1)(tile_z+1) x 8
  *     bytes, fits in LDS (160 KB).  Default 16 x 32 x 32.
  *   fft_variant: radix ordering of the transforms: 0 = radix 8 first (the
- *     fewest stages, default), 1 = the same radices with the leftover 4, 2,
- *     3, 5 first, 2 = radix 4 first (no radix 8).
+ *     fewest stages; the FP32 default), 1 = the same radices with the
+ *     leftover 4, 2, 3, 5 first, 2 = radix 4 first (no radix 8), 3 = one
+ *     radix 4 first in the y and x passes, ordering 0 in the z pass (the
+ *     FP64 default).
  * Changing the tiles drops the plan's deposits and its kept tile order. */
 int genpk_plan_set_options(genpk_plan *plan, int tile_x, int tile_y, int tile_z, int fft_variant);
 /* The plan's options in effect (tile edges after clamping to dims). */
 int genpk_plan_get_options(const genpk_plan *plan, int *tile_x, int *tile_y, int *tile_z, int *fft_variant);
 /* The radices, in stage order, of a length-n transform under fft_variant
- * (host only; the y and x passes use n = dims, the z passes n = dims/2). */
+ * (host only; the y and x passes use n = dims, the z passes n = dims/2,
+ * with ordering 0 under variant 3). */
 int genpk_radix_plan(int64_t n, int fft_variant, int *radices, int max, int *count);
 
 /* Empty the density (the reference zeroes the grid, gen-pk.cpp:208). */

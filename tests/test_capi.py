@@ -93,11 +93,15 @@ def test_radix_orderings():
     assert genpk_amd.radix_plan(1024, 1) == [2, 8, 8, 8]
     assert genpk_amd.radix_plan(1024, 2) == [4, 4, 4, 4, 4]
     assert genpk_amd.radix_plan(384, 1) == [2, 3, 8, 8]
+    assert genpk_amd.radix_plan(1024, 3) == [4, 8, 8, 4]
+    assert genpk_amd.radix_plan(512, 3) == [4, 8, 8, 2]
     for n in (2, 6, 10, 48, 100, 192, 240, 384, 512, 1000):
-        for v in range(3):
+        for v in range(4):
             r = genpk_amd.radix_plan(n, v)
             assert int(np.prod(r)) == n and set(r) <= {2, 3, 4, 5, 8}
             if v == 1:
                 assert sorted(r) == sorted(genpk_amd.radix_plan(n, 0))
+            if v == 3 and n % 4 == 0:
+                assert r[0] == 4
     with pytest.raises(genpk_amd.GenPKError):
         genpk_amd.radix_plan(14, 0)

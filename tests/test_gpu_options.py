@@ -27,7 +27,7 @@ def to_dev(torch, a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-13), ("f32", 3e-6)])
 @pytest.mark.parametrize("d", [48, 128, 384, 512])
 def test_fft_variants_vs_numpy(gpu, d, prec, tol, variant):
@@ -71,7 +71,7 @@ def test_deposit_tiles_vs_oracle(gpu, d, with_masses, prec, tiles):
     assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("tiles", [(16, 32, 32), (8, 16, 32), (4, 8, 16)])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("d", [128, 384])
@@ -150,7 +150,7 @@ def test_adjoint_with_options(gpu, tiles, variant):
 def test_options_errors(gpu):
     announce("options errors")
     p = genpk_amd.Plan(64, "f64")
-    for bad in [((16, 32, 3), 0), ((16, 32, 1), 0), ((32, 32, 32), 0), ((16, 32, 32), 3)]:
+    for bad in [((16, 32, 3), 0), ((16, 32, 1), 0), ((32, 32, 32), 0), ((16, 32, 32), 4), ((16, 32, 32), -1)]:
         with pytest.raises(genpk_amd.GenPKError):
             p.set_options(*bad)
     p.set_options((64, 4, 2), 0)          # 65 x 5 x 3 cells: fits
