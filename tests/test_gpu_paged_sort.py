@@ -140,3 +140,39 @@ def test_paged_sort_with_masses(gpu):
     assert torch.isfinite(gp).all() and torch.isfinite(gm).all()
     assert torch.equal(gp[:h], gp_lo) and torch.equal(gp[h:], gp_hi)
     assert torch.equal(gm[:h], gm_lo) and torch.equal(gm[h:], gm_hi)
+
+
+@pytest.mark.gpu
+def test_deposit_beyond_2_31_particles(gpu):
+    """2^31 + 2^20 particles on a 512^3 FP64 grid (7- and 6-bit digits): the
+    deposit takes them in calls of at most 2^31 particles (the reference's
+    int64 segment_particles), the first through the paged pass.  Against
+    two deposits split elsewhere (2^30 and the rest), to the rounding of the
+    per-cell sums, and the total mass."""
+    import torch
+    n, d, box = (1 << 31) + (1 << 20), 512, 250.0
+    announce("deposit beyond 2^31", f"n=2^31+2^20 d={d} f64")
+    g = torch.Generator(device="cuda").manual_seed(13)
+    pos = torch.rand((n, 3), dtype=torch.float32, device="cuda", generator=g).mul_(box)
+    a = genpk_amd.Plan(d, "f64")
+    a.timing(True)
+    a.clear()
+    a.deposit(pos, box)
+    torch.cuda.synchronize()
+    st = _stages(a)
+    assert "sort_pages" in st, st
+    ref = a.export_density().clone()
+    del a
+    torch.cuda.empty_cache()
+    b = genpk_amd.Plan(d, "f64")
+    b.clear()
+    h = 1 << 30
+    b.deposit(pos[:h], box)
+    b.deposit(pos[h:], box)
+    got = b.export_density()
+    diff = (got - ref).abs()
+    print(f"cells differing {(diff > 0).sum().item()}, max rel "
+          f"{(diff / ref.abs().clamp_min(1e-300)).max().item():.3e}", flush=True)
+    assert (diff <= 2.0 ** -50 * ref.abs()).all()
+    total = ref[:, :, :d].sum(dtype=torch.float64).item()
+    assert abs(total - n) <= 1e-9 * n
