@@ -40,8 +40,10 @@ template <> struct FftCfg<double> { static constexpr int ELEMS = 4096; static co
 // 8): each thread loads the eight inputs of up to two butterflies of the
 // first stage straight from the tile store and runs that stage from
 // registers.  CT = 512: the plan is M = 512 = 8 * 8 * 8 (d = 1024), run with
-// compile-time stages (fft_stage_ct); 0: any plan.
-template <typename T, bool FROM_TILES, bool BFLY = false, int CT = 0>
+// compile-time stages (fft_stage_ct); 0: any plan.  STD (with BFLY and CT):
+// the default 16 x 32 x 32 tiles, whose store offsets (below 2^31 at d =
+// 1024) are computed in 32 bits from compile-time strides.
+template <typename T, bool FROM_TILES, bool BFLY = false, int CT = 0, bool STD = false>
 __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(1, 2))) k_r2c_z(T *__restrict__ field, const T *__restrict__ tiles,
                                                       FoldGeom fg, FastDiv fdd, int64_t rowlen, int64_t nrows, int L,
                                                       FastDiv fdM, FastDiv fdM1, RadixPlan rp,
@@ -68,7 +70,29 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
       const int b = u / NQ, j = u % NQ;
       if (b < L) {
         C w[8];
-        if (b < nl) {
+        if (STD && b < nl) {
+          static_assert(!STD || (BFLY && CT == 512), "standard tiles with the d = 1024 butterfly loader");
+          // d = 1024, tiles 16 x 32 x 32 (17 x 33 halo'd rows of 32 per tile):
+          // z = 2 (j + 64 r) lies in tile column tz = 4 r + j / 16 at lz = 2 (j % 16)
+          constexpr uint32_t CM = 17 * 33 * 32, CZ = 17 * 33;
+          const uint32_t row = (uint32_t)(row0 + b);
+          const uint32_t x = row >> 10, y = row & 1023u;
+          const uint32_t txy = (x >> 4) * 32u + (y >> 5), rr = (x & 15u) * 33u + (y & 31u);
+          const uint32_t jz = (uint32_t)j >> 4, lz = 2u * ((uint32_t)j & 15u);
+          const uint32_t base = (txy * 32u + jz) * CM + rr * 32u + lz;
+          const T *tb = tiles + base;
+#pragma unroll
+          for (int r = 0; r < 8; r++) w[r] = C{tb[r * 4 * CM], tb[r * 4 * CM + 1]};
+          if (lz == 0) {
+            // the previous tile along z's z-halo cell: tile column 4 r + jz - 1
+            // (for r = jz = 0 the last column, 31, periodically)
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+              const uint32_t tzp = (4u * r + jz + 31u) & 31u;
+              w[r].x = w[r].x + tiles[(uint32_t)fg.zoff + (txy * 32u + tzp) * CZ + rr];  // one rounding, as below
+            }
+          }
+        } else if (b < nl) {
           const int64_t row = row0 + b;
           const int x = (int)fdd.div((uint32_t)row), y = (int)(row - (int64_t)x * fg.d);
           const int64_t txy = (int64_t)(x >> fg.sx) * fg.nty + (y >> fg.sy);
@@ -493,7 +517,13 @@ static void r2c_z_impl(Plan &p, bool from_tiles) {
                     (int64_t)L * (M / 8) <= 2 * FFT_THREADS;
   // the d = 1024 plan of radices 8, 8, 8 runs compile-time stages
   const bool ct512 = M == 512 && rp.nst == 3 && rp.st[0].R == 8 && rp.st[1].R == 8 && rp.st[2].R == 8;
-  if (from_tiles && bfly && ct512) {
+  const bool std_tiles = p.d == 1024 && fg.TX == 16 && fg.TY == 32 && fg.TZ == 32;
+  if (from_tiles && bfly && ct512 && std_tiles) {
+    Stage st(p, "fold_fft_z");
+    hipLaunchKernelGGL((k_r2c_z<T, true, true, 512, true>), grid, dim3(FFT_THREADS), lds, p.stream, p.field.as<T>(),
+                       p.tiles.as<const T>(), fg, FastDiv((uint32_t)p.d), p.rowlen, nrows, L, FastDiv((uint32_t)M),
+                       FastDiv((uint32_t)(M + 1)), rp, twM, twd);
+  } else if (from_tiles && bfly && ct512) {
     Stage st(p, "fold_fft_z");
     hipLaunchKernelGGL((k_r2c_z<T, true, true, 512>), grid, dim3(FFT_THREADS), lds, p.stream, p.field.as<T>(),
                        p.tiles.as<const T>(), fg, FastDiv((uint32_t)p.d), p.rowlen, nrows, L, FastDiv((uint32_t)M),
