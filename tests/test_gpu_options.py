@@ -106,7 +106,8 @@ def test_options_1024_fp32(gpu):
     d, box, n = 1024, 1000.0, 1 << 21
     pos = synth.clumped_torch(n, box, d, n_clumps=200, seed=61, device="cuda")
     res = {}
-    for tiles, variant in [((16, 32, 32), 0), ((8, 16, 32), 0), ((16, 32, 32), 2), ((8, 32, 32), 1)]:
+    for tiles, variant in [((16, 32, 32), 0), ((8, 16, 32), 0), ((16, 32, 32), 2), ((8, 32, 32), 1),
+                           ((16, 32, 32), 3)]:
         p = genpk_amd.Plan(d, "f32")
         p.set_options(tiles, variant)
         p.deposit(pos, box)
@@ -119,6 +120,35 @@ def test_options_1024_fp32(gpu):
     for key, (P, c) in res.items():
         assert np.array_equal(c, c0), key
         assert np.abs(P[occ] / P0[occ] - 1).max() <= 2e-5, key
+
+
+def test_options_1024_fp64(gpu):
+    """d = 1024, FP64: the default plan (radix ordering 3, one radix 4 first
+    in the column passes, the z pass with compile-time stages) against
+    ordering 0 (compile-time column stages, the fused x pass without the
+    register first stage) and another tile shape: equal counts, P within
+    1e-10."""
+    import torch
+    from genpk_amd import synth
+    announce("options d=1024 f64")
+    d, box, n = 1024, 1000.0, 1 << 21
+    pos = synth.clumped_torch(n, box, d, n_clumps=200, seed=62, device="cuda")
+    res = {}
+    for tiles, variant in [((16, 32, 32), None), ((16, 32, 32), 0), ((8, 32, 32), 3), ((16, 16, 32), 1)]:
+        p = genpk_amd.Plan(d, "f64")
+        if variant is not None:
+            p.set_options(tiles, variant)
+        assert p.options()["fft_variant"] == (3 if variant is None else variant)
+        p.deposit(pos, box)
+        P, c, _ = p.power(d, mass=float(n))
+        res[(tiles, variant)] = (P.cpu().numpy(), c.cpu().numpy())
+        del p
+        torch.cuda.empty_cache()
+    P0, c0 = res[((16, 32, 32), None)]
+    occ = c0 > 0
+    for key, (P, c) in res.items():
+        assert np.array_equal(c, c0), key
+        assert np.abs(P[occ] / P0[occ] - 1).max() <= 1e-10, key
 
 
 @pytest.mark.parametrize("tiles,variant", [((8, 16, 32), 2), ((4, 8, 16), 1)])
