@@ -95,6 +95,12 @@ struct ChunkMap {
   }
 };
 
+// Write target of the downsweep's staging positions past a sub-chunk's last
+// record (the write-out issues the same stores on every path): many
+// workgroups may write it at once, and nothing reads it.
+__device__ float sort_sink[SORT_THREADS * 8 * 4];
+__device__ uint8_t sort_sink_digits[SORT_THREADS * 8];
+
 // Paged pass 1 (MODE 1 and 2 of k_downsweep, two-pass sorts of many
 // records): the pass-1 output pool is cut into blocks of PG_BLOCK record
 // slots, Q per pass-1 workgroup.
@@ -159,11 +165,6 @@ DigitSel digit_sel(const KeyGeom &g, int shift, int width) {
 __device__ __forceinline__ uint64_t lanemask_lt() {
   const int lane = threadIdx.x & 63;
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
-}
-
-// number of set bits of m below this lane
-__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
-  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
 // counts[tb + dg * ts] = number of records of chunk g with digit dg
@@ -250,49 +251,45 @@ __global__ void __launch_bounds__(SORT_THREADS) k_upsweep_bytes(const uint8_t *_
     counts[tb + (int64_t)i * ts] = hist[i];
 }
 
-// Rank of each of a lane's PL records among the wave's records of its digit
-// in this sub-chunk, from LDS atomic adds that return the old count on the
-// wave's count row (a wave whose records share one digit adds them with one
-// atomic).  Equal digits are ordered by the LDS unit's processing order of
-// one instruction's lanes and, where waves share a count row, by the order
-// in which they run: the order of the records within a tile is not
-// specified (the tile kernels do not depend on it).
-// With `two`, lo also receives each record's digit of ds2 (computed from the
-// same cells).
+// Rank of each of a lane's PL record slots among the wave's slots of its
+// digit in this sub-chunk, from LDS atomic adds that return the old count on
+// the wave's count row (a wave whose 64 slots share one digit adds them with
+// one atomic, from lane 0).  Slots it >= nv hold no record: they take the
+// digit `trash` (one past the last), so that they rank and stage like
+// records, after every real one.  Equal digits are ordered by the LDS unit's
+// processing order of one instruction's lanes and, where waves share a count
+// row, by the order in which they run: the order of the records within a
+// tile is not specified (the tile kernels do not depend on it).  pk
+// receives each slot's rank (bits 0-12, below the sub-chunk's 8192 slots),
+// its digit from bit 13 and, with `two` (digits of at most 8 bits, so the
+// digit takes bits 13-21), its digit of ds2 (from the same key) from bit 22:
+// one register per slot.  Called by whole waves.
 template <int NM, int PL>
-__device__ __forceinline__ void rank_records(const float (&rec)[PL][Src<NM>::RW], const bool (&vld)[PL],
-                                             const KeyGeom &kg, const DigitSel &ds, const DigitSel &ds2, bool two,
-                                             uint32_t *wrow, uint32_t (&dig)[PL], uint32_t (&lo)[PL],
-                                             uint32_t (&rnk)[PL]) {
-  const int lane = threadIdx.x & 63;
-  uint64_t gpeers[PL];
-  int glead[PL];
+__device__ __forceinline__ void rank_records(const float (&rec)[PL][Src<NM>::RW], uint32_t nv, const KeyGeom &kg,
+                                             const DigitSel &ds, const DigitSel &ds2, bool two, uint32_t trash,
+                                             uint32_t *wrow, uint32_t (&pk)[PL]) {
+  const uint32_t lane = threadIdx.x & 63;
+  uint32_t rnk[PL];
+  uint32_t uni = 0;  // bit it: the wave's slots `it` share one digit
 #pragma unroll
   for (int it = 0; it < PL; it++) {
-    const bool valid = vld[it];
-    const uint32_t key = valid ? key_on_axes(rec[it], kg, two ? ds.axes | ds2.axes : ds.axes) : 0u;
-    const uint32_t dg = ds.of_key(key);
-    lo[it] = ds2.of_key(key);
-    const uint64_t vmask = __ballot(valid);
-    const int lead = (int)__builtin_ctzll(vmask | (1ull << 63));
-    const uint32_t d0 = __builtin_amdgcn_readlane(dg, lead);
+    const uint32_t key = tile_key(rec[it], kg);
+    const uint32_t dg = (uint32_t)it < nv ? ds.of_key(key) : trash;
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(dg);
     uint32_t r = 0;
-    if (__ballot(valid && dg == d0) == vmask) {
-      if (valid && lane == lead) r = atomicAdd(&wrow[d0], (uint32_t)__popcll(vmask));
-      gpeers[it] = vmask;
-      glead[it] = lead;
+    if (__ballot(dg == d0) == ~0ull) {
+      if (lane == 0) r = atomicAdd(&wrow[d0], 64u);
+      uni |= 1u << it;
     } else {
-      if (valid) r = atomicAdd(&wrow[dg], 1u);
-      gpeers[it] = 0;
-      glead[it] = -1;
+      r = atomicAdd(&wrow[dg], 1u);
     }
-    dig[it] = dg;
+    pk[it] = dg << 13 | (two ? ds2.of_key(key) << 22 : 0u);
     rnk[it] = r;
   }
-  // a one-digit wave's records follow its leader's old count
+  // a one-digit wave's slots follow lane 0's old count
 #pragma unroll
   for (int it = 0; it < PL; it++)
-    if (glead[it] >= 0) rnk[it] = __builtin_amdgcn_readlane(rnk[it], glead[it]) + mbcnt(gpeers[it]);
+    pk[it] |= (uni >> it) & 1u ? __builtin_amdgcn_readlane(rnk[it], 0) + lane : rnk[it];
 }
 
 // Per-row counts of digit dg -> per-row exclusive offsets in place; returns the total.
@@ -332,25 +329,29 @@ __device__ __forceinline__ uint32_t block_scan_round(uint32_t v, uint32_t (*wtot
 }
 
 // Partition of each chunk by digit.  perm (optional): perm[i] = output
-// position of input record i.  Per sub-chunk of SORT_THREADS * PL records
-// (each wave owns 64 * PL consecutive ones): ranks (rank_records), then one
+// position of input record i.  Per sub-chunk of SORT_THREADS * PL record
+// slots (each wave owns 64 * PL consecutive ones): ranks (rank_records; the
+// slots past the chunk's end take the digit nb, one past the last), then one
 // thread per digit turns the per-wave counts into offsets and a block scan
 // gives the digit's start in a digit-sorted staging copy and dbase = (global
 // output offset - staging start) mod 2^32, records go to the staging copy,
-// and the staging copy is written out as contiguous runs (record p goes to
-// dbase[its digit] + p) while the next sub-chunk's records load.  With DB
-// the per-wave counts are double-buffered: this sub-chunk's buffer is
-// cleared during the write-out and the next sub-chunk ranks into the other
-// one, so no barrier separates them.  hw count rows (a divisor of the wave
-// count) serve the waves, wave w on row w % hw.
+// and the staging copy's records (every slot before digit nb's start) are
+// written out as contiguous runs (record p goes to dbase[its digit] + p)
+// while the next sub-chunk's records load.  With DB the per-wave counts are
+// double-buffered: this sub-chunk's buffer is cleared during the write-out
+// and the next sub-chunk ranks into the other one, so no barrier separates
+// them.  hw count rows (a divisor of the wave count) serve the waves, wave w
+// on row w % hw.  A digit's staging start, global base, dbase and second
+// dbase sit together in one 16-byte LDS word (dinfo).
 //   MODE 0: the chunk's output offsets per digit are offsets[tb + dg * ts].
 //   MODE 1 (paged pass 1): workgroup g appends each digit's records to
 //     blocks of PG_BLOCK slots that it takes in order from its own range
 //     [g Q, g Q + Q) of the pool: the digit's current block until it is
-//     full, then new ones (a run then has two parts, split at gbase[dg]).
-//     Each block's digit and number among the digit's blocks go to pg.hdr,
-//     its fill to pg.fill, the number of blocks taken to pg.used[g].  No
-//     histogram pass and no global atomics.
+//     full, then new ones (a run then has two parts, the second from
+//     staging position dinfo[dg].y on, at dbase dinfo[dg].w).  Each block's
+//     digit and number among the digit's blocks go to pg.hdr, its fill to
+//     pg.fill, the number of blocks taken to pg.used[g].  No histogram pass
+//     and no global atomics.
 //   MODE 2 (paged pass 2): the chunk covers list slots of its segment's
 //     blocks (pg.blist); slot v is record (block of blist[v / PG_BLOCK], v %
 //     PG_BLOCK) when below that block's fill, and is skipped otherwise.
@@ -367,13 +368,12 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
   constexpr int NBUF = DB ? 2 : 1;
   constexpr uint32_t NONE = 0xffffffffu;
   static_assert(MODE != 2 || PL * 64 <= (int)PG_BLOCK, "a wave's records of a sub-chunk lie in one block");
-  extern __shared__ uint32_t sm[];
-  uint32_t *whist_all = sm;                                  // [NBUF][hw][nb]
-  uint32_t *lstart = whist_all + NBUF * hw * nb;             // [nb]
-  uint32_t *gbase = lstart + nb;                             // [nb] (MODE 1: first staging position of a run's second part)
-  uint32_t *dbase = gbase + nb;                              // [nb]
-  uint32_t *dbase2 = dbase + nb;                             // MODE 1 [nb]: dbase of a run's second part
-  uint32_t *cur_blk = dbase2 + (MODE == 1 ? nb : 0);         // MODE 1 [nb]: the digit's current block
+  static_assert(SUBN <= 8192, "ranks take 13 bits (rank_records)");
+  extern __shared__ uint4 sm4[];
+  const int nbt = nb + 1;                                    // digits and the empty slots' digit nb
+  uint4 *dinfo = sm4;                                        // [nbt] staging start, gbase, dbase, dbase2
+  uint32_t *whist_all = reinterpret_cast<uint32_t *>(dinfo + nbt);  // [NBUF][hw][nbt]
+  uint32_t *cur_blk = whist_all + NBUF * hw * nbt;           // MODE 1 [nb]: the digit's current block
   uint32_t *cur_fill = cur_blk + (MODE == 1 ? nb : 0);       // MODE 1 [nb]: its fill (PG_BLOCK: none open)
   uint32_t *dblocks = cur_fill + (MODE == 1 ? nb : 0);       // MODE 1 [nb]: blocks the digit took
   uint16_t *sdig = reinterpret_cast<uint16_t *>(dblocks + (MODE == 1 ? nb : 0));  // [SUBN]
@@ -384,10 +384,11 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
   int64_t cb, ce, tb, ts;
   if (!cm.range(blockIdx.x, cb, ce, tb, ts)) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // with dig2 (digits of at most 8 bits), sdig holds the digit and the ds2 digit
-  const bool two = dig2 != nullptr;
+  // with dig2 (digits of at most 8 bits), sdig holds the digit and the ds2
+  // digit (paged pass 1 always writes dig2, paged pass 2 never)
+  const bool two = MODE == 1 ? true : MODE == 2 ? false : dig2 != nullptr;
   if (MODE != 1)
-    for (int i = threadIdx.x; i < nb; i += SORT_THREADS) gbase[i] = offsets[tb + (int64_t)i * ts];
+    for (int i = threadIdx.x; i < nb; i += SORT_THREADS) dinfo[i].y = offsets[tb + (int64_t)i * ts];
   if (MODE == 1)
     for (int i = threadIdx.x; i < nb; i += SORT_THREADS) {
       cur_blk[i] = NONE;
@@ -396,16 +397,16 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
     }
   const uint32_t blk0 = MODE == 1 ? (uint32_t)(blockIdx.x * pg.Q) : 0u;  // this workgroup's block range
   uint32_t taken = 0;                                                    // MODE 1: blocks taken so far
-  for (int i = threadIdx.x; i < NBUF * hw * nb; i += SORT_THREADS) whist_all[i] = 0;
-  const int row = (wave % hw) * nb;
+  for (int i = threadIdx.x; i < NBUF * hw * nbt; i += SORT_THREADS) whist_all[i] = 0;
+  const int row = (wave % hw) * nbt;
   if (threadIdx.x == 0) {
     smr[0] = 0u;
     smr[1] = 0xffffffffu;
   }
   uint32_t mx = 0u, mn = 0xffffffffu;
 
-  float rec[PL][RW];
-  bool vld[PL];  // record it of this lane is valid
+  float rec[PL][RW] = {};
+  uint32_t nv = 0;  // this lane's slots of the sub-chunk that hold records (a prefix: it < nv)
   const int wbase = wave * (PL * 64) + lane;
   // MODE 2: a wave's 64 * PL consecutive list slots (PL * 64 <= PG_BLOCK,
   // sub-chunks start at multiples of PG_BLOCK) lie in one block: its pool
@@ -413,57 +414,60 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
   uint32_t wblk = 0, wfill = 0;
   // records of the sub-chunk at s0 (MODE 2: through the block list)
   auto load_sub = [&](int64_t s0) {
+    const int64_t i0 = s0 + wbase;
+    int64_t lim = ce - i0;  // slot i0 + 64 it holds a record iff 64 it < lim
     if constexpr (MODE == 2) {
-      const int64_t i0 = s0 + wave * (PL * 64);
+      // the wave's first slot as a uniform value: the entry is a scalar load
+      const int64_t w0 = s0 + __builtin_amdgcn_readfirstlane(wave) * (PL * 64);
       wfill = 0;
-      if (i0 < ce) {
-        const uint32_t ent = __builtin_amdgcn_readfirstlane(pg.blist[i0 >> PG_LOG]);
+      if (w0 < ce) {
+        const uint32_t ent = pg.blist[w0 >> PG_LOG];
         wblk = pg_blk(ent);
         wfill = pg_fill(ent);
       }
+      lim = min(lim, (int64_t)wfill - (int64_t)((uint32_t)i0 & (PG_BLOCK - 1)));
     }
+    nv = lim <= 0 ? 0u : (uint32_t)min((int64_t)PL, (lim + 63) >> 6);
 #pragma unroll
     for (int it = 0; it < PL; it++) {
-      const int64_t i = s0 + wbase + it * 64;
-      if constexpr (MODE == 2) {
-        const uint32_t off = (uint32_t)i & (PG_BLOCK - 1);
-        vld[it] = i < ce && off < wfill;
-        if (vld[it]) src.load(wblk * PG_BLOCK + off, rec[it]);
-      } else {
-        vld[it] = i < ce;
-        if (vld[it]) src.load(i, rec[it]);
+      const int64_t i = i0 + it * 64;
+      if ((uint32_t)it < nv) {
+        if constexpr (MODE == 2) src.load(wblk * PG_BLOCK + ((uint32_t)i & (PG_BLOCK - 1)), rec[it]);
+        else src.load(i, rec[it]);
       }
     }
   };
   load_sub(cb);
+  // nothing in flight on entry to the loop, so that the wait for a
+  // sub-chunk's records there is not a wait for the write-out's stores
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
 
   int sub = 0;
   for (int64_t s0 = cb; s0 < ce; s0 += SUBN, sub++) {
-    uint32_t *whist = whist_all + (DB ? (sub & 1) * hw * nb : 0);
-    uint32_t dig[PL], lo[PL], rnk[PL];
-    rank_records<NM, PL>(rec, vld, kg, ds, ds2, two, whist + row, dig, lo, rnk);
+    uint32_t *whist = whist_all + (DB ? (sub & 1) * hw * nbt : 0);
+    uint32_t pk[PL];
+    rank_records<NM, PL>(rec, nv, kg, ds, ds2, two, (uint32_t)nb, whist + row, pk);
     __syncthreads();
-    uint32_t total = 0;  // records of this sub-chunk
     {
-      for (int r0 = 0, rd = 0; r0 < nb; r0 += SORT_THREADS, rd++) {
+      uint32_t total = 0;  // slots of this sub-chunk (MODE 1: and blocks taken << 16)
+      for (int r0 = 0, rd = 0; r0 < nbt; r0 += SORT_THREADS, rd++) {
         const int dg = r0 + threadIdx.x;
-        const uint32_t tot = dg < nb ? wave_offsets(whist, nb, hw, dg) : 0u;
+        const uint32_t tot = dg < nbt ? wave_offsets(whist, nbt, hw, dg) : 0u;
         if constexpr (MODE == 1) {
           // the run fills the current block (a records), then `need` new
           // blocks; counts and block numbers scan together (nb <= 256, one
-          // round: at most SUBN records and SUBN / PG_BLOCK + nb blocks)
-          const uint32_t cf = dg < nb ? cur_fill[dg] : PG_BLOCK;
+          // round: at most SUBN slots and SUBN / PG_BLOCK + nb blocks)
+          const bool real = dg < nb;
+          const uint32_t cf = real ? cur_fill[dg] : PG_BLOCK;
           const uint32_t a = min(tot, PG_BLOCK - cf);
-          const uint32_t need = (tot - a + PG_BLOCK - 1) >> PG_LOG;
+          const uint32_t need = real ? (tot - a + PG_BLOCK - 1) >> PG_LOG : 0u;
           const uint32_t run = block_scan_round(tot | need << 16, wtot, rd, total);
-          if (dg < nb) {
+          if (real) {
             const uint32_t lr = run & 0xffffu, first = blk0 + taken + (run >> 16);
             const uint32_t cbk = cur_blk[dg];
-            lstart[dg] = lr;
-            dbase[dg] = cbk * PG_BLOCK + cf - lr;  // used only when a > 0, i.e. cbk is open
-            gbase[dg] = lr + a;
-            dbase2[dg] = first * PG_BLOCK - (lr + a);
+            // dbase used only when a > 0, i.e. cbk is open
+            dinfo[dg] = make_uint4(lr, lr + a, cbk * PG_BLOCK + cf - lr, first * PG_BLOCK - (lr + a));
             if (need) {
               if (cbk != NONE) pg.fill[cbk] = PG_BLOCK;  // filled by this run
               const uint32_t nd = dblocks[dg];
@@ -477,37 +481,42 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
             } else {
               cur_fill[dg] = cf + tot;
             }
+          } else if (dg == nb) {
+            dinfo[nb].x = run & 0xffffu;
           }
         } else {
           const uint32_t run = block_scan_round(tot, wtot, rd, total);
           if (dg < nb) {
-            const uint32_t g = gbase[dg];
-            lstart[dg] = run;
-            dbase[dg] = g - run;
-            gbase[dg] = g + tot;
+            const uint32_t g = dinfo[dg].y;
+            dinfo[dg].x = run;
+            dinfo[dg].y = g + tot;
+            dinfo[dg].z = g - run;
+          } else if (dg == nb) {
+            dinfo[nb].x = run;
           }
         }
       }
-      if constexpr (MODE == 1) {
-        taken += total >> 16;
-        total &= 0xffffu;
-      }
+      if constexpr (MODE == 1) taken += total >> 16;
     }
     __syncthreads();
     // output position of the record at staging position p of digit d
     auto dest_of = [&](uint32_t p, uint32_t d) -> uint32_t {
-      if constexpr (MODE == 1) return p < gbase[d] ? dbase[d] + p : dbase2[d] + p;
-      else return dbase[d] + p;
+      if constexpr (MODE == 1) {
+        const uint4 di = dinfo[d];
+        return p < di.y ? di.z + p : di.w + p;
+      } else {
+        return dinfo[d].z + p;
+      }
     };
 #pragma unroll
     for (int it = 0; it < PL; it++) {
       const int loc = wbase + it * 64;
-      if (vld[it]) {
-        const uint32_t dg = dig[it];
-        const uint32_t p = lstart[dg] + whist[row + dg] + rnk[it];
+      const uint32_t dg = two ? (pk[it] >> 13) & 0x1ffu : pk[it] >> 13;
+      const uint32_t p = dinfo[dg].x + whist[row + dg] + (pk[it] & 0x1fffu);
 #pragma unroll
-        for (int k = 0; k < RW; k++) stage[p * RW + k] = rec[it][k];
-        sdig[p] = (uint16_t)(two ? dg | lo[it] << 8 : dg);
+      for (int k = 0; k < RW; k++) stage[p * RW + k] = rec[it][k];
+      sdig[p] = (uint16_t)(two ? dg | (pk[it] >> 22) << 8 : dg);
+      if (dg < (uint32_t)nb) {
         if constexpr (NM == 1) {
           // |mass| as bits orders like the value (a NaN above every finite one)
           const uint32_t mb = __float_as_uint(rec[it][3]) & 0x7fffffffu;
@@ -521,29 +530,52 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
     // the next sub-chunk loads during the barrier and the write-out
     load_sub(s0 + SUBN);
     __syncthreads();
-    auto put = [&](int p) {
+    const uint32_t nrec = dinfo[nb].x;  // records of this sub-chunk: staging positions [0, nrec)
+    // record_p: staging position p holds a record (p < nrec); the others
+    // are written to the sink
+    auto put = [&](int p, bool record_p) {
       const uint32_t sd = sdig[p];
-      const uint32_t dest = dest_of((uint32_t)p, two ? sd & 0xffu : sd);
+      const uint32_t dest = dest_of((uint32_t)p, record_p ? (two ? sd & 0xffu : sd) : 0u);
       float v[RW];
 #pragma unroll
       for (int k = 0; k < RW; k++) v[k] = stage[p * RW + k];
+      float *to = record_p ? dst + (int64_t)dest * RW : sort_sink + p * RW;
 #pragma unroll
-      for (int k = 0; k < RW; k++) dst[(int64_t)dest * RW + k] = v[k];
-      if (two) dig2[dest] = (uint8_t)(sd >> 8);
+      for (int k = 0; k < RW; k++) to[k] = v[k];
+      if (two) *(record_p ? dig2 + dest : sort_sink_digits + p) = (uint8_t)(sd >> 8);
     };
-    if (total == SUBN) {
-      if constexpr (MODE == 1) {
-        // two-part runs: fewer writes in flight keep the registers in bounds
+    // With the sink (paged passes; pass 1 only with masses, where it measured
+    // faster), every lane stores for each of its staging positions, so that
+    // every path issues the same stores and the next sub-chunk's loads
+    // (issued before them) are waited for without waiting for these writes;
+    // elsewhere the extra registers cost more than that wait.  Pass 1's
+    // two-part runs write two positions at a time to keep the registers in
+    // bounds.
+    constexpr bool SINK = MODE == 2 || (MODE == 1 && NM == 1);
+    if constexpr (SINK && MODE == 1) {
 #pragma unroll 2
-        for (int it = 0; it < PL; it++) put(threadIdx.x + it * SORT_THREADS);
+      for (int it = 0; it < PL; it++) {
+        const int p = threadIdx.x + it * SORT_THREADS;
+        put(p, p < (int)nrec);
+      }
+    } else if constexpr (SINK) {
+#pragma unroll
+      for (int it = 0; it < PL; it++) {
+        const int p = threadIdx.x + it * SORT_THREADS;
+        put(p, p < (int)nrec);
+      }
+    } else if (nrec == SUBN) {
+      if constexpr (MODE == 1) {
+#pragma unroll 2
+        for (int it = 0; it < PL; it++) put(threadIdx.x + it * SORT_THREADS, true);
       } else {
 #pragma unroll
-        for (int it = 0; it < PL; it++) put(threadIdx.x + it * SORT_THREADS);
+        for (int it = 0; it < PL; it++) put(threadIdx.x + it * SORT_THREADS, true);
       }
     } else {
-      for (int p = threadIdx.x; p < (int)total; p += SORT_THREADS) put(p);
+      for (int p = threadIdx.x; p < (int)nrec; p += SORT_THREADS) put(p, true);
     }
-    for (int i = threadIdx.x; i < hw * nb; i += SORT_THREADS) whist[i] = 0;
+    for (int i = threadIdx.x; i < hw * nbt; i += SORT_THREADS) whist[i] = 0;
     if (!DB) __syncthreads();
   }
   if constexpr (MODE == 1) {
@@ -778,10 +810,11 @@ struct DsShape {
   int hw;
 };
 
-// LDS of a downsweep shape; MODE 1 keeps four more arrays per digit.
+// LDS of a downsweep shape (nb digits and the empty slots' digit); MODE 1
+// keeps three more arrays per digit.
 size_t downsweep_lds(int nb, int rw, const DsShape &h, int mode = 0) {
-  const size_t sub = (size_t)SORT_THREADS * h.pl;
-  return (size_t)((h.db ? 2 : 1) * h.hw * nb + (mode == 1 ? 7 : 3) * nb) * 4 + sub * 2 + sub * rw * 4;
+  const size_t sub = (size_t)SORT_THREADS * h.pl, nbt = (size_t)nb + 1;
+  return ((4 + (h.db ? 2 : 1) * h.hw) * nbt + (mode == 1 ? 3 * (size_t)nb : 0)) * 4 + sub * 2 + sub * rw * 4;
 }
 
 // The first shape whose LDS fits one workgroup per CU: four count rows (each

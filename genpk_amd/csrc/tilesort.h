@@ -50,12 +50,22 @@ __device__ __forceinline__ int cell_of_x86(float p, double units, int d, double 
   return c;
 }
 
+// Tile key of a position: cell_of's cells along the three axes, with one
+// branch for the rare wrap of any of them into [0, d).
 __device__ __forceinline__ uint32_t tile_key(const float *r, const KeyGeom &g) {
-  double fr;
-  const int cx = cell_of(r[0], g.units, g.d, fr);
-  const int cy = cell_of(r[1], g.units, g.d, fr);
-  const int cz = cell_of(r[2], g.units, g.d, fr);
-  return (uint32_t)((((cx >> g.sx) * g.nty + (cy >> g.sy)) * g.ntz) + (cz >> g.sz));
+  int c[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) c[a] = (int)floor((double)r[a] * g.units);
+  const unsigned ud = (unsigned)g.d;
+  if (((unsigned)c[0] >= ud) | ((unsigned)c[1] >= ud) | ((unsigned)c[2] >= ud)) {
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+      if ((unsigned)c[a] >= ud) {
+        c[a] %= g.d;
+        if (c[a] < 0) c[a] += g.d;
+      }
+  }
+  return (uint32_t)((((c[0] >> g.sx) * g.nty + (c[1] >> g.sy)) * g.ntz) + (c[2] >> g.sz));
 }
 
 // Work item -> tile k and its particle range [pb, pe): tile k has m items
