@@ -428,6 +428,8 @@ __global__ void __launch_bounds__(COL_THREADS) k_cols_r1(typename Cx<T>::C *__re
   __syncthreads();
   if constexpr (CT == 1024 && L2 && R == 8)
     lds_fft_ct<1024, 8, SIGN, true, R, false, C, false, 8, 8>(buf, buf, B, lgb, stw);
+  else if constexpr (CT == 1024 && !L2 && R == 4)
+    lds_fft_ct<1024, 4, SIGN, true, R, false, C, false, 8, 8, 4>(buf, buf, B, lgb, stw);
   else
     for (int s = 1; s < rp.nst - (L2 ? 1 : 0); s++)
       fft_stage_r<SIGN, true, R, false, C, false>(rp.st[s].R, buf, buf, B, lgb, B, 1, rp.st[s], stw);
@@ -586,6 +588,9 @@ static void cols_impl(Plan &p, int axis) {
     else if (rp.st[rp.nst - 1].R == 2)
       hipLaunchKernelGGL((k_cols_r1<T, SIGN, true>), dim3((unsigned)nblk), dim3(COL_THREADS), lds, p.stream,
                          p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, axis, rp, tw);
+    else if (FftCfg<T>::COLE == 4 && is_plan_4884(rp))
+      hipLaunchKernelGGL((k_cols_r1<T, SIGN, false, FftCfg<T>::COLE == 4 ? 1024 : 0>), dim3((unsigned)nblk),
+                         dim3(COL_THREADS), lds, p.stream, p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, axis, rp, tw);
     else
       hipLaunchKernelGGL((k_cols_r1<T, SIGN, false>), dim3((unsigned)nblk), dim3(COL_THREADS), lds, p.stream,
                          p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, axis, rp, tw);

@@ -203,6 +203,12 @@ inline bool is_plan_8882(const RadixPlan &p) {
   return p.n == 1024 && p.nst == 4 && p.st[0].R == 8 && p.st[1].R == 8 && p.st[2].R == 8 && p.st[3].R == 2;
 }
 
+// The FP64 column-pass plan that runs compile-time stages: n = 1024 as
+// radices 4, 8, 8, 4 (d = 1024, radix ordering 3, the FP64 default).
+inline bool is_plan_4884(const RadixPlan &p) {
+  return p.n == 1024 && p.nst == 4 && p.st[0].R == 4 && p.st[1].R == 8 && p.st[2].R == 8 && p.st[3].R == 4;
+}
+
 inline bool fft_length_ok(int64_t n) {
   if (n < 1 || n > (1 << 20)) return false;
   return make_radix_plan((int)n).nst >= 0;

@@ -539,10 +539,13 @@ __global__ void __launch_bounds__(SORT_THREADS, 4) k_downsweep(Src<NM> src, floa
       float v[RW];
 #pragma unroll
       for (int k = 0; k < RW; k++) v[k] = stage[p * RW + k];
-      float *to = record_p ? dst + (int64_t)dest * RW : sort_sink + p * RW;
+      // the sink position from p computed in the loop (not hoisted out of it)
+      int ps = p;
+      asm volatile("" : "+v"(ps));
+      float *to = record_p ? dst + (int64_t)dest * RW : sort_sink + ps * RW;
 #pragma unroll
       for (int k = 0; k < RW; k++) to[k] = v[k];
-      if (two) *(record_p ? dig2 + dest : sort_sink_digits + p) = (uint8_t)(sd >> 8);
+      if (two) *(record_p ? dig2 + dest : sort_sink_digits + ps) = (uint8_t)(sd >> 8);
     };
     // With the sink (paged passes; pass 1 only with masses, where it measured
     // faster), every lane stores for each of its staging positions, so that
