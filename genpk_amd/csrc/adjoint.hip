@@ -79,15 +79,10 @@ __global__ void k_weight(typename Cx<T>::C *__restrict__ f, int64_t d, int64_t h
 
 // x pass of the adjoint on the field after the forward z and y passes:
 // per column block (fixed y, B consecutive kz), forward length-d FFT along
-// x, multiply by the adjoint weight, inverse FFT, store back.  PP (n * B =
-// ADJ_THREADS * MAXE and a first stage of radix MAXE): the first stage runs
-// from the loaded registers and the stages alternate between two LDS
-// buffers (one barrier each).
-// LT (with PP and a last radix 2): the per-bin factors 2 lambda_b / (M^2 C_b),
-// the window factors and the bin thresholds are staged in LDS; each thread's
-// modes find their bin by tracking the thresholds along kx, so a mode's
-// weight costs no global read and no division.
-template <typename T, bool PP, bool LT = false>
+// x, multiply by the adjoint weight, inverse FFT, store back.  One block
+// per workgroup, the stages in place in one LDS buffer (k_adjoint_x_pp
+// covers the usual shapes).
+template <typename T>
 __global__ void __launch_bounds__(ADJ_THREADS) k_adjoint_x(typename Cx<T>::C *__restrict__ f, int64_t d, int64_t hz,
                                                           int lgb, int nkb, RadixPlan rp,
                                                           const typename Cx<T>::C *__restrict__ tw, AdjWeight w) {
@@ -105,98 +100,11 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_adjoint_x(typename Cx<T>::C *__
   const int64_t estride = d * hz;
   C *base = f + y * hz + kz0;
   const int tot = n * B;
-  C v[MAXE];
 #pragma unroll
   for (int q = 0; q < MAXE; q++) {
     const int i = threadIdx.x + q * ADJ_THREADS;
     const int e = i >> lgb, b = i & (B - 1);
-    v[q] = (i < tot && b < nb) ? base[e * estride + b] : C{0, 0};
-  }
-  if constexpr (!PP) {
-#pragma unroll
-    for (int q = 0; q < MAXE; q++) {
-      const int i = threadIdx.x + q * ADJ_THREADS;
-      if (i < tot) buf[i] = v[q];
-    }
-  }
-  if constexpr (PP) {
-    C *buf2 = buf + tot;
-    C *stw = buf2 + tot;
-    for (int i = threadIdx.x; i < n; i += ADJ_THREADS) stw[i] = tw[i];
-    double *sfb = reinterpret_cast<double *>(stw + n);  // [nrbins]
-    float *swt = reinterpret_cast<float *>(sfb + w.nrbins);  // [d]
-    int *sthr = reinterpret_cast<int *>(swt + d);  // [nrbins + 1]
-    if constexpr (LT) {
-      for (int i = threadIdx.x; i < w.nrbins; i += ADJ_THREADS) {
-        const int c = w.count[i];
-        sfb[i] = c ? 2.0 * w.lambda[i] * w.inv_m2 / (double)c : 0.0;
-      }
-      for (int i = threadIdx.x; i < d; i += ADJ_THREADS) swt[i] = w.wtab[i];
-      for (int i = threadIdx.x; i <= w.nrbins; i += ADJ_THREADS) sthr[i] = (int)w.thr[i];
-    }
-    fft_first_regs<MAXE, -1>(v, buf2, B, lgb, B, 1);
-    __syncthreads();
-    C *res;
-    if (rp.st[rp.nst - 1].R == 2) {
-      // the last forward stage (radix 2) and the adjoint weight in registers:
-      // butterfly jj pairs stage inputs jj and jj + n/2 (the latter times
-      // w^jj) into modes jj and jj + n/2, written weighted to the other buffer
-      C *x = lds_fft_pp<-1, true, MAXE>(buf2, buf, rp, B, lgb, B, 1, stw, 1, rp.nst - 1);
-      res = x == buf ? buf2 : buf;
-      constexpr int HB = MAXE / 2;
-      const int half = n / 2;
-      const int b = threadIdx.x & (B - 1);
-      const int jb = (threadIdx.x >> lgb) * HB;
-      // LT: modes jj (kx = jj) and jj + n/2 (kx = jj - n/2, kx = n/2 at jj = 0)
-      // each keep a bin tracker along their run
-      const int kz = (int)kz0 + b, ky = (int)kv(y, d);
-      const int a = ky * ky + kz * kz;
-      const float wy = LT ? swt[y] : 0.f, wz = LT ? swt[min(kz, (int)d - 1)] : 0.f;
-      BinTrackLds t0, t1;
-      auto lds_weight = [&](int xi, BinTrackLds &bt) -> double {
-        const int kx = xi <= half ? xi : xi - n;
-        const int qk = a + kx * kx;
-        if (qk == 0) return 0.0;
-        const float gw = (swt[xi] * wy) * wz;  // the forward's ((wx*wy)*wz) in float
-        double g = (double)gw * (double)gw;
-        g *= g;
-        return g * sfb[bt.of(qk, sthr, w.nrbins, w.lq)];
-      };
-#pragma unroll
-      for (int q = 0; q < HB; q++) {
-        const int jj = jb + q;
-        const C a0 = x[jj * B + b];
-        const C v1 = cmul(x[(jj + half) * B + b], stw[jj]);
-        C o0 = cadd(a0, v1), o1 = csub(a0, v1);
-        if (b < nb) {
-          const double s0 = LT ? lds_weight(jj, t0) : w.at(jj, y, kz0 + b, d);
-          const double s1 = LT ? lds_weight(jj + half, t1) : w.at(jj + half, y, kz0 + b, d);
-          o0 = C{(T)((double)o0.x * s0), (T)((double)o0.y * s0)};
-          o1 = C{(T)((double)o1.x * s1), (T)((double)o1.y * s1)};
-        }
-        res[jj * B + b] = o0;
-        res[(jj + half) * B + b] = o1;
-      }
-    } else {
-      res = lds_fft_pp<-1, true, MAXE>(buf2, buf, rp, B, lgb, B, 1, stw, 1);
-      for (int i = threadIdx.x; i < tot; i += ADJ_THREADS) {
-        const int e = i >> lgb, b = i & (B - 1);
-        if (b < nb) {
-          const double s = w.at(e, y, kz0 + b, d);
-          const C a = res[i];
-          res[i] = C{(T)((double)a.x * s), (T)((double)a.y * s)};
-        }
-      }
-    }
-    __syncthreads();
-    res = lds_fft_pp<+1, true, MAXE>(res, res == buf ? buf2 : buf, rp, B, lgb, B, 1, stw);
-#pragma unroll
-    for (int q = 0; q < MAXE; q++) {
-      const int i = threadIdx.x + q * ADJ_THREADS;
-      const int e = i >> lgb, b = i & (B - 1);
-      if (i < tot && b < nb) base[e * estride + b] = res[i];
-    }
-    return;
+    if (i < tot) buf[i] = b < nb ? base[e * estride + b] : C{0, 0};
   }
   C *stw = buf + tot;
   for (int i = threadIdx.x; i < n; i += ADJ_THREADS) stw[i] = tw[i];
@@ -218,6 +126,135 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_adjoint_x(typename Cx<T>::C *__
     const int i = threadIdx.x + q * ADJ_THREADS;
     const int e = i >> lgb, b = i & (B - 1);
     if (i < tot && b < nb) base[e * estride + b] = buf[i];
+  }
+}
+
+// k_adjoint_x for n * B = ADJ_THREADS * MAXE and a first radix of MAXE,
+// persistent over the column blocks (v, v + gridDim.x, ...): the first
+// stage runs from the loaded registers, the stages alternate between two
+// LDS buffers (one barrier each), and the next block's loads are in flight
+// during this block's inverse transform and stores.
+// LT (a last radix 2): the last forward stage runs in registers with the
+// weight; the per-bin factors 2 lambda_b / (M^2 C_b), the window factors
+// and the bin thresholds are staged in LDS, and each thread's modes find
+// their bin by tracking the thresholds along kx, so a mode's weight costs
+// no global read and no division.  (Compile-time stages for n = 1024 spill
+// registers here: 5.2 -> 7.3 ms at 1024^3, against 4.3 ms without.)
+template <typename T, bool LT>
+__global__ void __launch_bounds__(ADJ_THREADS) k_adjoint_x_pp(typename Cx<T>::C *__restrict__ f, int64_t d,
+                                                             int64_t hz, int lgb, int nkb, RadixPlan rp,
+                                                             const typename Cx<T>::C *__restrict__ tw, AdjWeight w) {
+  using C = typename Cx<T>::C;
+  constexpr int MAXE = AdjCfg<T>::MAXE;
+  extern __shared__ unsigned char lds_raw[];
+  const int n = rp.n;
+  const int B = 1 << lgb;
+  const int tot = n * B;
+  C *buf = reinterpret_cast<C *>(lds_raw);  // [n][B]
+  C *buf2 = buf + tot;
+  C *stw = buf2 + tot;
+  for (int i = threadIdx.x; i < n; i += ADJ_THREADS) stw[i] = tw[i];
+  double *sfb = reinterpret_cast<double *>(stw + n);  // [nrbins]
+  float *swt = reinterpret_cast<float *>(sfb + w.nrbins);  // [d]
+  int *sthr = reinterpret_cast<int *>(swt + d);  // [nrbins + 1]
+  if constexpr (LT) {
+    for (int i = threadIdx.x; i < w.nrbins; i += ADJ_THREADS) {
+      const int c = w.count[i];
+      sfb[i] = c ? 2.0 * w.lambda[i] * w.inv_m2 / (double)c : 0.0;
+    }
+    for (int i = threadIdx.x; i < d; i += ADJ_THREADS) swt[i] = w.wtab[i];
+    for (int i = threadIdx.x; i <= w.nrbins; i += ADJ_THREADS) sthr[i] = (int)w.thr[i];
+  }
+  const int64_t estride = d * hz;
+  const int64_t nblk = d * nkb;
+  const int b = threadIdx.x & (B - 1);
+  C v[MAXE];
+  // column block vb's loads: the inputs of this thread's first-stage butterfly
+  // (a grid that is a multiple of the XCD count keeps vb on this workgroup's XCD)
+  auto load = [&](int64_t vb) {
+    const uint32_t blk = xcd_block((uint32_t)vb, (uint32_t)nblk);
+    const int64_t y = blk / nkb;
+    const int64_t kz0 = (int64_t)(blk - y * nkb) * B;
+    const int nb = (int)min((int64_t)B, hz - kz0);
+    const C *src = f + y * hz + kz0;
+#pragma unroll
+    for (int q = 0; q < MAXE; q++) {
+      const int i = threadIdx.x + q * ADJ_THREADS;
+      const int e = i >> lgb;
+      v[q] = b < nb ? src[e * estride + b] : C{0, 0};
+    }
+  };
+  if ((int64_t)blockIdx.x < nblk) load(blockIdx.x);
+  for (int64_t vb = blockIdx.x; vb < nblk; vb += gridDim.x) {
+    const uint32_t blk = xcd_block((uint32_t)vb, (uint32_t)nblk);
+    const int64_t y = blk / nkb;
+    const int64_t kz0 = (int64_t)(blk - y * nkb) * B;
+    const int nb = (int)min((int64_t)B, hz - kz0);
+    C *base = f + y * hz + kz0;
+    // the previous block's stores have read their buffer before this barrier
+    fft_first_regs<MAXE, -1>(v, buf2, B, lgb, B, 1);
+    __syncthreads();
+    C *res;
+    if (LT) {
+      // the last forward stage (radix 2) and the adjoint weight in registers:
+      // butterfly jj pairs stage inputs jj and jj + n/2 (the latter times
+      // w^jj) into modes jj and jj + n/2, written weighted to the other buffer
+      C *x = lds_fft_pp<-1, true, MAXE>(buf2, buf, rp, B, lgb, B, 1, stw, 1, rp.nst - 1);
+      res = x == buf ? buf2 : buf;
+      constexpr int HB = MAXE / 2;
+      const int half = n / 2;
+      const int jb = (threadIdx.x >> lgb) * HB;
+      // modes jj (kx = jj) and jj + n/2 (kx = jj - n/2, kx = n/2 at jj = 0)
+      // each keep a bin tracker along their run
+      const int kz = (int)kz0 + b, ky = (int)kv(y, d);
+      const int a = ky * ky + kz * kz;
+      const float wy = swt[y], wz = swt[min(kz, (int)d - 1)];
+      BinTrackLds t0, t1;
+      auto lds_weight = [&](int xi, BinTrackLds &bt) -> double {
+        const int kx = xi <= half ? xi : xi - n;
+        const int qk = a + kx * kx;
+        if (qk == 0) return 0.0;
+        const float gw = (swt[xi] * wy) * wz;  // the forward's ((wx*wy)*wz) in float
+        double g = (double)gw * (double)gw;
+        g *= g;
+        return g * sfb[bt.of(qk, sthr, w.nrbins, w.lq)];
+      };
+#pragma unroll
+      for (int q = 0; q < HB; q++) {
+        const int jj = jb + q;
+        const C a0 = x[jj * B + b];
+        const C v1 = cmul(x[(jj + half) * B + b], stw[jj]);
+        C o0 = cadd(a0, v1), o1 = csub(a0, v1);
+        if (b < nb) {
+          const double s0 = lds_weight(jj, t0);
+          const double s1 = lds_weight(jj + half, t1);
+          o0 = C{(T)((double)o0.x * s0), (T)((double)o0.y * s0)};
+          o1 = C{(T)((double)o1.x * s1), (T)((double)o1.y * s1)};
+        }
+        res[jj * B + b] = o0;
+        res[(jj + half) * B + b] = o1;
+      }
+    } else {
+      res = lds_fft_pp<-1, true, MAXE>(buf2, buf, rp, B, lgb, B, 1, stw, 1);
+      for (int i = threadIdx.x; i < tot; i += ADJ_THREADS) {
+        const int e = i >> lgb;
+        if (b < nb) {
+          const double s = w.at(e, y, kz0 + b, d);
+          const C a = res[i];
+          res[i] = C{(T)((double)a.x * s), (T)((double)a.y * s)};
+        }
+      }
+    }
+    __syncthreads();
+    // the next block's loads (v is free since the first stage)
+    if (vb + gridDim.x < nblk) load(vb + gridDim.x);
+    res = lds_fft_pp<+1, true, MAXE>(res, res == buf ? buf2 : buf, rp, B, lgb, B, 1, stw);
+#pragma unroll
+    for (int q = 0; q < MAXE; q++) {
+      const int i = threadIdx.x + q * ADJ_THREADS;
+      const int e = i >> lgb;
+      if (b < nb) base[e * estride + b] = res[i];
+    }
   }
 }
 
@@ -394,15 +431,25 @@ void adjoint_x_impl(Plan &p, const AdjWeight &w) {
   const size_t lds_lt = (size_t)(2 * tot + p.d) * sizeof(C) + (size_t)w.nrbins * 8 + (size_t)p.d * 4 +
                         (size_t)(w.nrbins + 1) * 4;
   const bool lt = pp && rp.st[rp.nst - 1].R == 2 && w.nrbins >= 1 && lds_lt <= 160 * 1024;
-  if (lt)
-    hipLaunchKernelGGL((k_adjoint_x<T, true, true>), grid, dim3(ADJ_THREADS), lds_lt, p.stream, p.field.as<C>(), p.d,
-                       p.hz, ilog2(B), nkb, rp, tw, w);
-  else if (pp)
-    hipLaunchKernelGGL((k_adjoint_x<T, true>), grid, dim3(ADJ_THREADS), (size_t)(2 * tot + p.d) * sizeof(C), p.stream,
+  if (pp) {
+    // persistent: as many workgroups as stay resident, at most one per column block
+    // (4.3 ms at 1024^3 against 5.2 ms for one workgroup per block)
+    const void *fn = lt ? reinterpret_cast<const void *>(&k_adjoint_x_pp<T, true>)
+                        : reinterpret_cast<const void *>(&k_adjoint_x_pp<T, false>);
+    const size_t lds = lt ? lds_lt : (size_t)(2 * tot + p.d) * sizeof(C);
+    int per_cu = 0;
+    GENPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, ADJ_THREADS, lds));
+    const dim3 g((unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)std::max(per_cu, 1) * num_cus(), p.d * nkb)));
+    if (lt)
+      hipLaunchKernelGGL((k_adjoint_x_pp<T, true>), g, dim3(ADJ_THREADS), lds, p.stream, p.field.as<C>(), p.d, p.hz,
+                         ilog2(B), nkb, rp, tw, w);
+    else
+      hipLaunchKernelGGL((k_adjoint_x_pp<T, false>), g, dim3(ADJ_THREADS), lds, p.stream, p.field.as<C>(), p.d, p.hz,
+                         ilog2(B), nkb, rp, tw, w);
+  } else {
+    hipLaunchKernelGGL((k_adjoint_x<T>), grid, dim3(ADJ_THREADS), (size_t)(tot + p.d) * sizeof(C), p.stream,
                        p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, rp, tw, w);
-  else
-    hipLaunchKernelGGL((k_adjoint_x<T, false>), grid, dim3(ADJ_THREADS), (size_t)(tot + p.d) * sizeof(C), p.stream,
-                       p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, rp, tw, w);
+  }
   GENPK_LAUNCH_CHECK();
 }
 
