@@ -27,6 +27,10 @@ constexpr int SORT_WAVES = SORT_THREADS / 64;
 // downsweep) up to SORT_CHUNK_MAX, the smallest that keeps at most four
 // chunks per CU (small sets then still spread over every CU).
 constexpr int64_t SORT_CHUNK_MIN = 8192, SORT_CHUNK_MAX = 65536;
+// The paged second pass's count kernel: four waves per workgroup, so that
+// eight workgroups (chunks) per CU overlap their memory round trips.
+constexpr int UP_THREADS = 256;
+constexpr int UP_WAVES = UP_THREADS / 64;
 
 // Source of records: either the caller's arrays (positions [n][3], optional
 // masses [n]) or a packed record buffer of RW floats per record.
@@ -709,25 +713,26 @@ __global__ void k_pg_seginfo(const uint32_t *__restrict__ boff, const uint32_t *
 // the listed blocks' filled slots.  Chunks start at multiples of PG_BLOCK
 // list slots; each wave takes whole blocks (one lookup of the block and its
 // fill), each lane 16 consecutive slots of it.
-__global__ void __launch_bounds__(SORT_THREADS) k_upsweep_pages(const uint8_t *__restrict__ digs, ChunkMap cm,
+__global__ void __launch_bounds__(UP_THREADS) k_upsweep_pages(const uint8_t *__restrict__ digs, ChunkMap cm,
                                                                 PgArgs pg, int nb, uint32_t *__restrict__ counts) {
   static_assert(PG_BLOCK == 64 * 16, "one block per wave, 16 slots per lane");
   extern __shared__ uint32_t hist[];
   int64_t b, e, tb, ts;
   if (!cm.range(blockIdx.x, b, e, tb, ts)) return;
-  for (int i = threadIdx.x; i < nb; i += SORT_THREADS) hist[i] = 0;
+  for (int i = threadIdx.x; i < nb; i += UP_THREADS) hist[i] = 0;
   __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wave as a uniform value: the block-list entries are scalar loads
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int64_t k0 = b >> PG_LOG, k1 = (e + PG_BLOCK - 1) >> PG_LOG;  // list entries of the chunk
   constexpr int UNR = 4;  // blocks per wave in flight
-  for (int64_t kb = k0 + wave; kb < k1; kb += UNR * SORT_WAVES) {
+  for (int64_t kb = k0 + wave; kb < k1; kb += UNR * UP_WAVES) {
     uint32_t blk[UNR];
     int nv[UNR];
     uint4 v[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; u++) {
-      const int64_t k = kb + u * SORT_WAVES;
-      const uint32_t ent = k < k1 ? __builtin_amdgcn_readfirstlane(pg.blist[k]) : 0u;
+      const int64_t k = kb + u * UP_WAVES;
+      const uint32_t ent = k < k1 ? pg.blist[k] : 0u;
       blk[u] = pg_blk(ent);
       // filled slots of this block inside the chunk, from this lane's 16
       const int64_t lim = k < k1 ? min((int64_t)pg_fill(ent), e - (k << PG_LOG)) : 0;
@@ -762,7 +767,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_upsweep_pages(const uint8_t *_
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nb; i += SORT_THREADS)
+  for (int i = threadIdx.x; i < nb; i += UP_THREADS)
     counts[tb + (int64_t)i * ts] = hist[i];
 }
 
@@ -1032,7 +1037,7 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
     {
       Stage st(p, "sort_upsweep2");
       if (L.on)
-        hipLaunchKernelGGL(k_upsweep_pages, dim3((unsigned)g2), dim3(SORT_THREADS), nlo * 4, s, dig2, cm2, pg, nlo,
+        hipLaunchKernelGGL(k_upsweep_pages, dim3((unsigned)g2), dim3(UP_THREADS), nlo * 4, s, dig2, cm2, pg, nlo,
                            t2);
       else if (dig2)
         hipLaunchKernelGGL(k_upsweep_bytes, dim3((unsigned)g2), dim3(SORT_THREADS), nlo * 4, s, dig2, cm2, nlo, t2);
