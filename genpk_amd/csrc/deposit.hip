@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
   }
   // a cell's value at the flush, which clears the cell for the next item
   // (every cell is read once, by one thread)
-  auto cell = [&](int64_t c) -> double {
+  auto cell = [&](int c) -> double {
     const Acc a = acc[c];
     acc[c] = 0;
     if constexpr (NM == 1)
@@ -134,6 +134,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
     const int ty = (int)((k / ntz) % nty);
     const int tx = (int)(k / ((int64_t)ntz * nty));
     const int ox = tx * TX, oy = ty * TY, oz = tz * TZ;
+    const int wx = min(TX, P.kg.d - ox), wy = min(TY, P.kg.d - oy), wz = min(TZ, P.kg.d - oz);  // the tile's own cells
     for (uint32_t i0 = pb; i0 < pe; i0 += BATCH) {
 #pragma unroll
       for (int u = 0; u < UNR; u++)
@@ -145,12 +146,29 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
       for (int u = 0; u < UNR; u++) {
         const uint32_t i = i0 + u * DEP_THREADS + threadIdx.x;
         if (i >= pe) continue;
+        // cell_of without its wrap: a floor cell inside the tile's own cells
+        // [o, o + w) lies in [0, d), where cell_of does not wrap
         double fx, fy, fz;
-        const int lx = cell_of(r[u][0], P.kg.units, P.kg.d, fx) - ox;
-        const int ly = cell_of(r[u][1], P.kg.units, P.kg.d, fy) - oy;
-        const int lz = cell_of(r[u][2], P.kg.units, P.kg.d, fz) - oz;
-        if ((unsigned)lx >= (unsigned)TX || (unsigned)ly >= (unsigned)TY || (unsigned)lz >= (unsigned)TZ)
-          continue;  // cannot happen for records of this tile; guards the LDS bounds
+        int lx, ly, lz;
+        {
+          const double ux = (double)r[u][0] * P.kg.units, uy = (double)r[u][1] * P.kg.units;
+          const double uz = (double)r[u][2] * P.kg.units;
+          const double flx = floor(ux), fly = floor(uy), flz = floor(uz);
+          fx = ux - flx;
+          fy = uy - fly;
+          fz = uz - flz;
+          lx = (int)flx - ox;
+          ly = (int)fly - oy;
+          lz = (int)flz - oz;
+        }
+        if ((unsigned)lx >= (unsigned)wx || (unsigned)ly >= (unsigned)wy || (unsigned)lz >= (unsigned)wz) {
+          // a position outside [0, box) (its cells wrap), or a non-finite one
+          lx = cell_of(r[u][0], P.kg.units, P.kg.d, fx) - ox;
+          ly = cell_of(r[u][1], P.kg.units, P.kg.d, fy) - oy;
+          lz = cell_of(r[u][2], P.kg.units, P.kg.d, fz) - oz;
+          if ((unsigned)lx >= (unsigned)TX || (unsigned)ly >= (unsigned)TY || (unsigned)lz >= (unsigned)TZ)
+            continue;  // cannot happen for records of this tile; guards the LDS bounds
+        }
         if (!(fx + fy + fz <= 3.0)) {
           // a non-finite position: k_poison sets its eight cells to NaN
           *P.nonfinite = 1u;
@@ -159,7 +177,8 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
         const double gx = 1.0 - fx, gy = 1.0 - fy, gz = 1.0 - fz;
         const double wxy[4] = {gx * gy, fx * gy, gx * fy, fx * fy};
         const double gzs = gz * FIXED_ONE, fzs = fz * FIXED_ONE;  // exact: a power-of-two scale
-        const int base = lx * sx + ly * sy + lz;
+        // 24-bit products (one full-rate instruction each; every factor is below 2^24)
+        const int base = (int)(__umul24((unsigned)lx, (unsigned)sx) + __umul24((unsigned)ly, (unsigned)sy)) + lz;
         const int off[4] = {0, sx, sy, sx + sy};
 #pragma unroll
         for (int c = 0; c < 4; c++) {
@@ -188,22 +207,21 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
     if (next) load_batch(nir.pb, nir.pe);
     if (m == 1) {
       // rows of TZ values (aligned), then the z-halo plane
+      // (32-bit cell indices: a tile's accumulator holds fewer than 2^24 cells)
       Real *rows = tiles + k * P.cm;
       Real *zh = tiles + P.zoff + k * P.cz;
-      const int sz = P.kg.sz;
+      const int sz = STD ? 5 : P.kg.sz, hz = STD ? 33 : P.HZ;
+      const int cm = (int)P.cm, cz = (int)P.cz;
       if (P.store) {
-        for (int64_t c = threadIdx.x; c < P.cm; c += DEP_THREADS) {
-          const int64_t a2 = (c >> sz) * P.HZ + (c & (TZ - 1));
-          rows[c] = (Real)cell(a2);
-        }
-        for (int64_t c = threadIdx.x; c < P.cz; c += DEP_THREADS) zh[c] = (Real)cell(c * P.HZ + TZ);
+#pragma unroll 4
+        for (int c = threadIdx.x; c < cm; c += DEP_THREADS)
+          rows[c] = (Real)cell((int)__umul24((unsigned)(c >> sz), (unsigned)hz) + (c & (TZ - 1)));
+        for (int c = threadIdx.x; c < cz; c += DEP_THREADS) zh[c] = (Real)cell((int)__umul24((unsigned)c, (unsigned)hz) + TZ);
       } else {
-        for (int64_t c = threadIdx.x; c < P.cm; c += DEP_THREADS) {
-          const int64_t a2 = (c >> sz) * P.HZ + (c & (TZ - 1));
-          rows[c] = (Real)((double)rows[c] + cell(a2));
-        }
-        for (int64_t c = threadIdx.x; c < P.cz; c += DEP_THREADS)
-          zh[c] = (Real)((double)zh[c] + cell(c * P.HZ + TZ));
+        for (int c = threadIdx.x; c < cm; c += DEP_THREADS)
+          rows[c] = (Real)((double)rows[c] + cell((int)__umul24((unsigned)(c >> sz), (unsigned)hz) + (c & (TZ - 1))));
+        for (int c = threadIdx.x; c < cz; c += DEP_THREADS)
+          zh[c] = (Real)((double)zh[c] + cell((int)__umul24((unsigned)c, (unsigned)hz) + TZ));
       }
     } else {
       // a split tile: this item's raw accumulators (fixed-point integers, or
