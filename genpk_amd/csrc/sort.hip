@@ -25,8 +25,11 @@ constexpr int SORT_THREADS = 1024;
 constexpr int SORT_WAVES = SORT_THREADS / 64;
 // Records per workgroup: from SORT_CHUNK_MIN (one sub-chunk of the
 // downsweep) up to SORT_CHUNK_MAX, the smallest that keeps at most four
-// chunks per CU (small sets then still spread over every CU).
-constexpr int64_t SORT_CHUNK_MIN = 8192, SORT_CHUNK_MAX = 65536;
+// chunks per CU (small sets then still spread over every CU).  32 Ki rather
+// than 64 Ki: the paged second pass's count kernel runs more, shorter
+// workgroups (2^27 particles at 512^3: 0.23 -> 0.17 ms) and the downsweeps
+// are unchanged or slightly faster; 16 Ki is no faster.
+constexpr int64_t SORT_CHUNK_MIN = 8192, SORT_CHUNK_MAX = 32768;
 // The paged second pass's count kernel: four waves per workgroup, so that
 // eight workgroups (chunks) per CU overlap their memory round trips.
 constexpr int UP_THREADS = 256;
