@@ -17,8 +17,11 @@ partitioned:
   pool, perm2 out of it) and a gather that partitions the particles itself;
 * redeposit() from the kept paged order.
 
-The clustered 1024^3 parity test against the FP64 host oracle runs the
-paged path at 2^30 particles.
+The particle counts are not whole numbers of the downsweep's 8192-record
+sub-chunks, so each pass's last sub-chunk is partial (its empty slots rank
+under the spare digit past the last one, and the paged passes write them to
+the sink).  The clustered 1024^3 parity test against the FP64 host oracle
+runs the paged path at 2^30 particles.
 """
 import pytest
 
@@ -37,9 +40,9 @@ def _stages(plan):
 @pytest.mark.gpu
 def test_paged_sort_matches_plain_halves(gpu):
     import torch
-    n, d, box = 1 << 29, 1024, 1000.0
+    n, d, box = (1 << 29) + 12345, 1024, 1000.0
     h = n // 2
-    announce("paged sort vs plain halves", f"n=2^29 d={d} f64")
+    announce("paged sort vs plain halves", f"n=2^29+12345 d={d} f64")
     pos = synth.clumped_torch(n, box, d, n_clumps=20000, seed=11)
     # the paged pass at n, the plain passes at n / 2
     a = genpk_amd.Plan(d, "f64")
@@ -99,9 +102,9 @@ def test_paged_sort_with_masses(gpu):
     fixed-point density and both gradients (positions and masses) against
     the plain halves."""
     import torch
-    n, d, box = 1 << 29, 1024, 1000.0
+    n, d, box = (1 << 29) + 54321, 1024, 1000.0
     h = n // 2
-    announce("paged sort with masses vs plain halves", f"n=2^29 d={d} f64")
+    announce("paged sort with masses vs plain halves", f"n=2^29+54321 d={d} f64")
     pos = synth.clumped_torch(n, box, d, n_clumps=20000, seed=12)
     m = torch.rand(n, dtype=torch.float32, device="cuda", generator=torch.Generator(device="cuda").manual_seed(6)) + 0.5
     a = genpk_amd.Plan(d, "f64")
