@@ -400,8 +400,9 @@ __global__ void __launch_bounds__(COL_THREADS) k_cols(typename Cx<T>::C *__restr
 // addresses as k_cols' element-wise load), so the first stage runs from
 // registers and the load's LDS round trip and two barriers are saved.
 // L2 (last radix 2): the last stage's butterflies (rows k and k + n/2 of a
-// column, twiddle w^k) write their outputs straight to global memory.  CT as
-// k_cols.
+// column, twiddle w^k) write their outputs straight to global memory.
+// CT = 1024: compile-time stages for n = 1024 as 8, 8, 8, 2 (FP32, with L2)
+// or 4, 8, 8, 4 (FP64 without L2).
 template <typename T, int SIGN, bool L2, int CT = 0>
 __global__ void __launch_bounds__(COL_THREADS) k_cols_r1(typename Cx<T>::C *__restrict__ f, int64_t d, int64_t hz,
                                                         int lgb, int nkb, int axis, RadixPlan rp,
