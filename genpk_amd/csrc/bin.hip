@@ -553,10 +553,10 @@ static size_t fused_lds(const Plan &a, int B, int nrbins, bool pp) {
          (size_t)a.d * 4 + (size_t)nrbins * 4;
 }
 
-// Whether the REG1 instance applies (one field with ping-pong buffers).
+// Whether the REG1 instance applies to plan rp (one field with ping-pong
+// buffers).
 template <typename T>
-static bool fused_reg1(const Plan &a, int B, bool pp) {
-  const RadixPlan &rp = a.rp_d;
+static bool fused_reg1(const Plan &a, int B, bool pp, const RadixPlan &rp) {
   return pp && rp.st[0].R == BinCfg<T>::MAXE && a.d * B == (int64_t)BIN_THREADS * BinCfg<T>::MAXE && rp.nst % 2 == 0;
 }
 
@@ -573,13 +573,13 @@ static int fused_grid(const Plan &a, int B, int nrbins, bool pp) {
 }
 
 template <typename T, int NF, bool REG1, bool R2L = false, int CT = 0>
-static void launch_fused(Plan &a, Plan *b, const BinTables &bt, int B, bool keep, bool pp, double *part, int G) {
+static void launch_fused(Plan &a, Plan *b, const BinTables &bt, const RadixPlan &rp, int B, bool keep, bool pp,
+                         double *part, int G) {
   using C = typename Cx<T>::C;
   constexpr int NS = NF == 1 ? 1 : 3;
   // the R2L instance finishes the transform only in registers for the binning
   GENPK_REQUIRE(!(R2L && keep), "the fused pass cannot keep the spectrum with the last stage in the binning");
   const int nkb = (int)ceil_div(a.hz, B);
-  const RadixPlan &rp = a.rp_d;
   const C *tw = (a.prec == 4) ? (const C *)a.fft.tw_d_f32.ptr : (const C *)a.fft.tw_d_f64.ptr;
   BinGeom g{a.d, a.hz, bt.nrbins, bt.q2bin.as<int>(), bt.thr.as<int64_t>(), a.wtab.as<float>(), bt.lq};
   const size_t lds = fused_lds<T, NF>(a, B, bt.nrbins, pp);
@@ -600,11 +600,12 @@ static void launch_stored(Plan &a, Plan *b, const BinTables &bt, double *part, i
   GENPK_LAUNCH_CHECK();
 }
 
-// Launch shape of the fused x pass + bin: column block B, ping-pong buffers
-// (pp), the register first stage (reg1), the radix-2 last stage in the
-// binning (r2l), compile-time stages for the n = 1024 plan 8, 8, 8, 2 (ct,
-// with r2l), and the persistent grid G.
+// Launch shape of the fused x pass + bin: the radix plan of its transforms
+// (rp), column block B, ping-pong buffers (pp), the register first stage
+// (reg1), the radix-2 last stage in the binning (r2l), compile-time stages
+// for the n = 1024 plan 8, 8, 8, 2 (ct, with r2l), and the persistent grid G.
 struct FusedShape {
+  RadixPlan rp;
   int B = 0;
   bool pp = false, reg1 = false, r2l = false, ct = false;
   int G = 0;
@@ -621,11 +622,11 @@ static int fused_grid_of(const Plan &a, bool cross, const FusedShape &s, int nrb
 
 template <typename T>
 static void launch_fused_of(Plan &a, Plan *b, const BinTables &bt, const FusedShape &s, double *part) {
-  if (b) launch_fused<T, 2, false>(a, b, bt, s.B, false, false, part, s.G);
-  else if (s.r2l && s.ct) launch_fused<T, 1, true, true, 1024>(a, b, bt, s.B, false, s.pp, part, s.G);
-  else if (s.r2l) launch_fused<T, 1, true, true>(a, b, bt, s.B, false, s.pp, part, s.G);
-  else if (s.reg1) launch_fused<T, 1, true>(a, b, bt, s.B, false, s.pp, part, s.G);
-  else launch_fused<T, 1, false>(a, b, bt, s.B, false, s.pp, part, s.G);
+  if (b) launch_fused<T, 2, false>(a, b, bt, s.rp, s.B, false, false, part, s.G);
+  else if (s.r2l && s.ct) launch_fused<T, 1, true, true, 1024>(a, b, bt, s.rp, s.B, false, s.pp, part, s.G);
+  else if (s.r2l) launch_fused<T, 1, true, true>(a, b, bt, s.rp, s.B, false, s.pp, part, s.G);
+  else if (s.reg1) launch_fused<T, 1, true>(a, b, bt, s.rp, s.B, false, s.pp, part, s.G);
+  else launch_fused<T, 1, false>(a, b, bt, s.rp, s.B, false, s.pp, part, s.G);
 }
 
 static FusedShape fused_shape(const Plan &a, const Plan *b, int nrbins) {
@@ -637,13 +638,24 @@ static FusedShape fused_shape(const Plan &a, const Plan *b, int nrbins) {
   // one field: ping-pong FFT buffers where the LDS has room for them
   s.pp = !b && (a.prec == 4 ? fused_lds<float, 1>(a, B, nrbins, true)
                             : fused_lds<double, 1>(a, B, nrbins, true)) <= (size_t)LDS_LIMIT;
-  s.reg1 = !b && (a.prec == 4 ? fused_reg1<float>(a, B, s.pp) : fused_reg1<double>(a, B, s.pp));
+  s.rp = a.rp_d;
+  s.reg1 = !b && (a.prec == 4 ? fused_reg1<float>(a, B, s.pp, s.rp) : fused_reg1<double>(a, B, s.pp, s.rp));
   // FP32 with a last stage of radix 2: that stage runs in the binning
-  const RadixPlan &xrp = a.rp_d;
-  s.r2l = s.reg1 && a.prec == 4 && xrp.st[xrp.nst - 1].R == 2;
+  s.r2l = s.reg1 && a.prec == 4 && s.rp.st[s.rp.nst - 1].R == 2;
+  if (!b && a.prec == 4 && !s.r2l) {
+    // the plan's own ordering (the column passes') does not start with a
+    // radix 8 and end with a radix 2 in an even number of stages: the x pass
+    // takes a plan of its length that does, where the register first stage
+    // applies (512 = 8 8 8 -> 8 8 4 2: 0.40 -> 0.30 ms at 512^3)
+    const RadixPlan xp = radix_plan_8_to_2((int)a.d);
+    if (xp.nst > 0 && fused_reg1<float>(a, B, s.pp, xp)) {
+      s.rp = xp;
+      s.reg1 = s.r2l = true;
+    }
+  }
   // compile-time stages where they pay: the FP32 register-first instance at
   // n = 1024 (the others would spill registers)
-  s.ct = s.r2l && is_plan_8882(xrp);
+  s.ct = s.r2l && is_plan_8882(s.rp);
   const bool cross = b != nullptr;
   s.G = a.prec == 4 ? fused_grid_of<float>(a, cross, s, nrbins) : fused_grid_of<double>(a, cross, s, nrbins);
   return s;

@@ -158,6 +158,25 @@ struct RadixPlan {
 constexpr int FFT_VARIANTS = 4;
 inline int default_fft_variant(int prec) { return prec == 8 ? 3 : 0; }
 
+// Stockham stage plans of the radix sequence r[0..nr) (product n).
+inline RadixPlan radix_plan_of(int n, const int *r, int nr) {
+  RadixPlan p;
+  p.n = n;
+  int ns = 1;
+  for (int s = 0; s < nr; s++) {
+    StagePlan &sp = p.st[s];
+    sp.R = r[s];
+    sp.ns = ns;
+    sp.nq = n / r[s];
+    sp.twstep = n / (ns * r[s]);
+    sp.fns = FastDiv((uint32_t)ns);
+    sp.fnq = FastDiv((uint32_t)sp.nq);
+    ns *= r[s];
+  }
+  p.nst = nr;
+  return p;
+}
+
 inline RadixPlan make_radix_plan(int n, int variant = 0) {
   RadixPlan p;
   p.n = n;
@@ -182,19 +201,29 @@ inline RadixPlan make_radix_plan(int n, int variant = 0) {
     for (int s = 0; s < e; s++) t[k++] = 8;
     for (int s = 0; s < nr; s++) r[s] = t[s];
   }
-  int ns = 1;
-  for (int s = 0; s < nr; s++) {
-    StagePlan &sp = p.st[s];
-    sp.R = r[s];
-    sp.ns = ns;
-    sp.nq = n / r[s];
-    sp.twstep = n / (ns * r[s]);
-    sp.fns = FastDiv((uint32_t)ns);
-    sp.fnq = FastDiv((uint32_t)sp.nq);
-    ns *= r[s];
-  }
-  p.nst = nr;
-  return p;
+  return radix_plan_of(n, r, nr);
+}
+
+// Plan of a power-of-two length n >= 16 with a first radix 8, an even number
+// of stages and a last radix 2, the fewest such stages (the middle radices
+// as even as possible, larger first): 16 = 8 2, 64 = 8 2 2 2, 128 = 8 4 2 2,
+// 256 = 8 4 4 2, 512 = 8 8 4 2, 1024 = 8 8 8 2.  nst = -1 for other lengths
+// and for 32.
+inline RadixPlan radix_plan_8_to_2(int n) {
+  RadixPlan p;
+  p.n = n;
+  p.nst = -1;
+  if (n < 16 || (n & (n - 1)) != 0) return p;
+  int t = -4;  // log2 of the middle stages' product
+  for (int m = n; m > 1; m >>= 1) t++;
+  int k = (t + 2) / 3;  // fewest middle stages of radix <= 8 ...
+  if (k % 2) k++;       // ... in an even number
+  if (k > t) return p;
+  int r[14], nr = 0;
+  r[nr++] = 8;
+  for (int i = 0; i < k; i++) r[nr++] = 1 << (t / k + (i < t % k ? 1 : 0));
+  r[nr++] = 2;
+  return radix_plan_of(n, r, nr);
 }
 
 // The column-pass plan that runs compile-time stages (fft_stage_ct): n =
