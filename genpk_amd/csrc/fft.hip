@@ -36,9 +36,9 @@ template <> struct FftCfg<double> { static constexpr int ELEMS = 4096; static co
 // register budget of two waves per SIMD is available.  The rows sit in LDS
 // with one element of padding per 32 (lds_ix), which spreads the strided
 // writes of the FFT stages over the banks.
-// BFLY (FP32, from the tile store, M == FFT_THREADS, a first stage of radix
-// 8): each thread loads the eight inputs of up to two butterflies of the
-// first stage straight from the tile store and runs that stage from
+// BFLY (FP32, from the tile store, a first stage of radix 8, L * M / 8 <= 2 *
+// FFT_THREADS): each thread loads the eight inputs of up to two butterflies
+// of the first stage straight from the tile store and runs that stage from
 // registers.  CT = 512: the plan is M = 512 = 8 * 8 * 8 (d = 1024), run with
 // compile-time stages (fft_stage_ct); 0: any plan.  STD (with BFLY and CT):
 // the default 16 x 32 x 32 tiles, whose store offsets (below 2^31 at d =
@@ -63,11 +63,11 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
   if constexpr (BFLY) {
     // butterfly u = threadIdx.x + FFT_THREADS * i: row u / NQ, inputs
     // u % NQ + NQ * r (NQ = M / 8)
-    constexpr int NQ = FFT_THREADS / 8;
+    const int NQ = M / 8;
 #pragma unroll
     for (int i = 0; i < 2; i++) {
       const int u = threadIdx.x + FFT_THREADS * i;
-      const int b = u / NQ, j = u % NQ;
+      const int b = CT ? u / (CT / 8) : (int)fdM.div((uint32_t)(8 * u)), j = u - b * NQ;
       if (b < L) {
         C w[8];
         if (STD && b < nl) {
@@ -227,15 +227,27 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
     }
     return;
   }
+  if (FFT_THREADS % M == 0) {
+    // M a smaller power of two: output k = threadIdx.x % M of rows
+    // threadIdx.x / M + FFT_THREADS / M * i, and k = M by the threads of k = 0
+    const int k = threadIdx.x & (M - 1), rpi = FFT_THREADS / M;
+    const C w = sxw[k], wm = sxw[M];
+#pragma unroll 4
+    for (int l = threadIdx.x / M; l < nl; l += rpi) {
+      const C zk = buf[lds_ix<true>(l * M + k)];
+      const C zc = cconj(buf[lds_ix<true>(l * M + (k == 0 ? 0 : M - k))]);
+      C *row = reinterpret_cast<C *>(field + (row0 + l) * rowlen);
+      row[k] = split(zk, zc, w);
+      if (k == 0) row[M] = split(zk, zc, wm);
+    }
+    return;
+  }
   const int nout = M + 1;
   for (int i = threadIdx.x; i < nl * nout; i += FFT_THREADS) {
     const int l = (int)fdM1.div((uint32_t)i), k = i - l * nout;
     const C zk = buf[lds_ix<true>(l * M + (k == M ? 0 : k))];
     const C zc = cconj(buf[lds_ix<true>(l * M + (k == 0 ? 0 : M - k))]);
-    const C x = split(zk, zc, sxw[k]);
-    T *row = field + (row0 + l) * rowlen;
-    row[2 * k] = x.x;
-    row[2 * k + 1] = x.y;
+    reinterpret_cast<C *>(field + (row0 + l) * rowlen)[k] = split(zk, zc, sxw[k]);
   }
 }
 
@@ -516,8 +528,7 @@ static void r2c_z_impl(Plan &p, bool from_tiles) {
   const FoldGeom fg = fold_geom(p);
   const dim3 grid((unsigned)ceil_div(nrows, L));
   const size_t lds = ((size_t)lds_len<true>((int64_t)L * M) + 2 * M + 1) * sizeof(C);
-  const bool bfly = sizeof(T) == 4 && M == FFT_THREADS && rp.st[0].R == 8 && rp.nst >= 2 &&
-                    (int64_t)L * (M / 8) <= 2 * FFT_THREADS;
+  const bool bfly = sizeof(T) == 4 && rp.st[0].R == 8 && rp.nst >= 2 && (int64_t)L * (M / 8) <= 2 * FFT_THREADS;
   // the d = 1024 plan of radices 8, 8, 8 runs compile-time stages
   const bool ct512 = M == 512 && rp.nst == 3 && rp.st[0].R == 8 && rp.st[1].R == 8 && rp.st[2].R == 8;
   const bool std_tiles = p.d == 1024 && fg.TX == 16 && fg.TY == 32 && fg.TZ == 32;
