@@ -133,10 +133,11 @@ struct BinTrack {
 // Fused final FFT pass along x + deconvolved binning.  NF fields (1: auto,
 // 2: auto a, auto b and cross a.b*).  Persistent over column blocks.
 //
-// REG1 (one field, ping-pong buffers, n * B = BIN_THREADS * MAXE, a first
-// stage of radix MAXE and an even number of stages): each thread's MAXE
-// loaded elements are the inputs of its butterfly in the first stage, which
-// then runs from registers into buf2.  The previous block no longer reads
+// REG1 (one field, ping-pong buffers, n * B <= BIN_THREADS * MAXE, a first
+// stage of radix MAXE and an even number of stages): the MAXE loaded
+// elements of each thread below n * B / MAXE are the inputs of its butterfly
+// in the first stage, which then runs from registers into buf2 (the threads
+// above it load nothing and bin nothing).  The previous block no longer reads
 // buf2 (an even number of stages leaves its result in buf), so no barrier
 // comes before that stage.
 //
@@ -175,9 +176,12 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
   const int my_b = threadIdx.x % B;
   const int per = (n + tpc - 1) / tpc;
   const int e0 = (threadIdx.x / B) * per;
-  // register-staged loads of column block v: all its global reads in flight together
+  // register-staged loads of column block v: all its global reads in flight
+  // together; with REG1 at a stride of the first stage's butterfly count
   constexpr int MAXE = BinCfg<T>::MAXE;
   const int tot = n * B;
+  const int lst = REG1 ? tot / MAXE : BIN_THREADS;
+  const bool loader = !REG1 || (int)threadIdx.x < lst;
   C va[MAXE], vb[NF == 2 ? MAXE : 1];
   auto load = [&](int64_t v) {
     // a grid that is a multiple of the XCD count keeps v on this workgroup's XCD
@@ -188,9 +192,9 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
     const int64_t off = y * hz + kz0;
 #pragma unroll
     for (int q = 0; q < MAXE; q++) {
-      const int i = threadIdx.x + q * BIN_THREADS;
+      const int i = threadIdx.x + q * lst;
       const int e = i >> lgb, b = i & (B - 1);
-      const bool ok = i < tot && b < nb;
+      const bool ok = loader && i < tot && b < nb;
       va[q] = ok ? fa[off + e * estride + b] : C{0, 0};
       if (NF == 2) vb[NF == 2 ? q : 0] = ok ? fb[off + e * estride + b] : C{0, 0};
     }
@@ -207,7 +211,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
     const int64_t off = y * hz + kz0;
     C *res = buf;
     if constexpr (REG1) {
-      fft_first_regs<MAXE, -1>(va, buf2, B, lgb, B, 1);
+      if (loader) fft_first_regs<MAXE, -1>(va, buf2, B, lgb, B, 1);
       __syncthreads();
       if constexpr (CT == 1024 && R2L && MAXE == 8)
         res = lds_fft_ct<1024, 8, -1, true, MAXE, true, C, false, 8, 8>(buf2, buf, B, lgb, stw);
@@ -257,8 +261,9 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
         // (the latter times w^jj) into modes jj and jj + n/2
         constexpr int HB = MAXE / 2;
         const int half = n / 2;
-        const int jb = (threadIdx.x / B) * HB;
+        const int jb = (threadIdx.x / B) * HB;  // below half exactly for the loaders
         C mo[MAXE];
+        if (loader) {
 #pragma unroll
         for (int q = 0; q < HB; q++) {
           const int jj = jb + q;
@@ -281,6 +286,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
             mode_terms<C, NF>(mo[q], mo[q], h, dc, c);
             run_add<NS>(r, bn, c, bins, g.nrbins);
           }
+        }
         }
         run_flush<NS>(r, bins, g.nrbins);
       } else {
@@ -554,10 +560,12 @@ static size_t fused_lds(const Plan &a, int B, int nrbins, bool pp) {
 }
 
 // Whether the REG1 instance applies to plan rp (one field with ping-pong
-// buffers).
+// buffers): every thread holds one first-stage butterfly, or (some_idle)
+// at most one.
 template <typename T>
-static bool fused_reg1(const Plan &a, int B, bool pp, const RadixPlan &rp) {
-  return pp && rp.st[0].R == BinCfg<T>::MAXE && a.d * B == (int64_t)BIN_THREADS * BinCfg<T>::MAXE && rp.nst % 2 == 0;
+static bool fused_reg1(const Plan &a, int B, bool pp, const RadixPlan &rp, bool some_idle = false) {
+  const int64_t tot = a.d * B, cap = (int64_t)BIN_THREADS * BinCfg<T>::MAXE;
+  return pp && rp.st[0].R == BinCfg<T>::MAXE && (some_idle ? tot <= cap : tot == cap) && rp.nst % 2 == 0;
 }
 
 // Persistent grid of the fused kernel: as many workgroups as stay resident
@@ -644,11 +652,12 @@ static FusedShape fused_shape(const Plan &a, const Plan *b, int nrbins) {
   s.r2l = s.reg1 && a.prec == 4 && s.rp.st[s.rp.nst - 1].R == 2;
   if (!b && a.prec == 4 && !s.r2l) {
     // the plan's own ordering (the column passes') does not start with a
-    // radix 8 and end with a radix 2 in an even number of stages: the x pass
-    // takes a plan of its length that does, where the register first stage
-    // applies (512 = 8 8 8 -> 8 8 4 2: 0.40 -> 0.30 ms at 512^3)
+    // radix 8 and end with a radix 2 in an even number of stages, or its
+    // columns leave threads idle: the x pass takes a plan of its length that
+    // does, where the register first stage applies (512 = 8 8 8 -> 8 8 4 2:
+    // 0.40 -> 0.30 ms at 512^3)
     const RadixPlan xp = radix_plan_8_to_2((int)a.d);
-    if (xp.nst > 0 && fused_reg1<float>(a, B, s.pp, xp)) {
+    if (xp.nst > 0 && fused_reg1<float>(a, B, s.pp, xp, true)) {
       s.rp = xp;
       s.reg1 = s.r2l = true;
     }

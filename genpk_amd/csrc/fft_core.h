@@ -204,24 +204,47 @@ inline RadixPlan make_radix_plan(int n, int variant = 0) {
   return radix_plan_of(n, r, nr);
 }
 
-// Plan of a power-of-two length n >= 16 with a first radix 8, an even number
-// of stages and a last radix 2, the fewest such stages (the middle radices
-// as even as possible, larger first): 16 = 8 2, 64 = 8 2 2 2, 128 = 8 4 2 2,
-// 256 = 8 4 4 2, 512 = 8 8 4 2, 1024 = 8 8 8 2.  nst = -1 for other lengths
-// and for 32.
+// Plan of a length n = 16 m with a first radix 8, an even number of stages
+// and a last radix 2.  Powers of two take the fewest such stages (the middle
+// radices as even as possible, larger first): 16 = 8 2, 64 = 8 2 2 2, 128 =
+// 8 4 2 2, 256 = 8 4 4 2, 512 = 8 8 4 2, 1024 = 8 8 8 2.  Other lengths take
+// ordering 0 of m in the middle, its first 8 (else 4) split in two where
+// that leaves an odd count: 384 = 8 8 3 2, 96 = 8 2 3 2.  nst = -1 where no
+// such plan exists (32, 48, lengths not divisible by 16).
 inline RadixPlan radix_plan_8_to_2(int n) {
   RadixPlan p;
   p.n = n;
   p.nst = -1;
-  if (n < 16 || (n & (n - 1)) != 0) return p;
-  int t = -4;  // log2 of the middle stages' product
-  for (int m = n; m > 1; m >>= 1) t++;
-  int k = (t + 2) / 3;  // fewest middle stages of radix <= 8 ...
-  if (k % 2) k++;       // ... in an even number
-  if (k > t) return p;
+  if (n < 16 || n % 16 != 0) return p;
   int r[14], nr = 0;
   r[nr++] = 8;
-  for (int i = 0; i < k; i++) r[nr++] = 1 << (t / k + (i < t % k ? 1 : 0));
+  if ((n & (n - 1)) == 0) {
+    int t = -4;  // log2 of the middle stages' product
+    for (int m = n; m > 1; m >>= 1) t++;
+    int k = (t + 2) / 3;  // fewest middle stages of radix <= 8 ...
+    if (k % 2) k++;       // ... in an even number
+    if (k > t) return p;
+    for (int i = 0; i < k; i++) r[nr++] = 1 << (t / k + (i < t % k ? 1 : 0));
+  } else {
+    const RadixPlan mp = make_radix_plan(n / 16, 0);
+    if (mp.nst < 0 || mp.nst > 10) return p;
+    int split = -1;
+    if (mp.nst % 2) {
+      for (int s = 0; s < mp.nst && split < 0; s++)
+        if (mp.st[s].R == 8) split = s;
+      for (int s = 0; s < mp.nst && split < 0; s++)
+        if (mp.st[s].R == 4) split = s;
+      if (split < 0) return p;
+    }
+    for (int s = 0; s < mp.nst; s++) {
+      if (s == split) {
+        r[nr++] = mp.st[s].R / 2;
+        r[nr++] = 2;
+      } else {
+        r[nr++] = mp.st[s].R;
+      }
+    }
+  }
   r[nr++] = 2;
   return radix_plan_of(n, r, nr);
 }
