@@ -36,7 +36,7 @@ template <> struct FftCfg<double> { static constexpr int ELEMS = 4096; static co
 // register budget of two waves per SIMD is available.  The rows sit in LDS
 // with one element of padding per 32 (lds_ix), which spreads the strided
 // writes of the FFT stages over the banks.
-// BFLY (FP32, from the tile store, a first stage of radix 8, L * M / 8 <= 2 *
+// BFLY (from the tile store, a first stage of radix 8, L * M / 8 <= 2 *
 // FFT_THREADS): each thread loads the eight inputs of up to two butterflies
 // of the first stage straight from the tile store and runs that stage from
 // registers.  CT = 512: the plan is M = 512 = 8 * 8 * 8 (d = 1024), run with
@@ -528,9 +528,12 @@ static void r2c_z_impl(Plan &p, bool from_tiles) {
   const FoldGeom fg = fold_geom(p);
   const dim3 grid((unsigned)ceil_div(nrows, L));
   const size_t lds = ((size_t)lds_len<true>((int64_t)L * M) + 2 * M + 1) * sizeof(C);
-  const bool bfly = sizeof(T) == 4 && rp.st[0].R == 8 && rp.nst >= 2 && (int64_t)L * (M / 8) <= 2 * FFT_THREADS;
   // the d = 1024 plan of radices 8, 8, 8 runs compile-time stages
   const bool ct512 = M == 512 && rp.nst == 3 && rp.st[0].R == 8 && rp.st[1].R == 8 && rp.st[2].R == 8;
+  // FP64 at d = 1024 (seven rows a workgroup) reads the rows element-wise:
+  // the butterfly loader measured 3.50 -> 3.73 ms there (0.52 -> 0.47 ms at 512)
+  const bool bfly = rp.st[0].R == 8 && rp.nst >= 2 && (int64_t)L * (M / 8) <= 2 * FFT_THREADS &&
+                    (sizeof(T) == 4 || !ct512);
   const bool std_tiles = p.d == 1024 && fg.TX == 16 && fg.TY == 32 && fg.TZ == 32;
   if (from_tiles && bfly && ct512 && std_tiles) {
     Stage st(p, "fold_fft_z");
