@@ -65,6 +65,8 @@ constexpr int FIXED_FRAC_BITS = 36;
 constexpr double FIXED_ONE = 68719476736.0;            // 2^36
 constexpr double FIXED_INV = 1.0 / 68719476736.0;      // 2^-36
 constexpr int DEPOSIT_ITEM_MAX = 131072;
+// Smallest item cap that deposit_item_max (tilesort.h) takes for few tiles.
+constexpr int DEPOSIT_ITEM_MIN = 16384;
 
 template <typename T> struct cplx_of;
 template <> struct cplx_of<float> { using type = float2; };

@@ -5,7 +5,8 @@
 // 2. Tile accumulation.  One workgroup per (tile, particle range) item
 //    accumulates the 8 CIC weights of each particle into the halo'd tile in
 //    LDS as 64-bit fixed point (ds_add_u64), then flushes the whole tile with
-//    coalesced stores.  Tiles with more than DEPOSIT_ITEM_MAX particles are
+//    coalesced stores.  Tiles with more particles than the item cap
+//    (deposit_item_max: DEPOSIT_ITEM_MAX, less when tiles are few) are
 //    split; their items write their raw accumulators as partial tiles, which a
 //    reduction sums exactly (fixed point) whatever records formed each item.
 // 3. Fold.  Grid cell values are assembled from the tile store by adding the
@@ -422,7 +423,7 @@ void deposit_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass) {
   const uint32_t *tile_start = o.tile_start, *item_start = o.item_start, *part_start = o.part_start;
   const uint32_t *nheavy = o.nheavy, *heavy = o.heavy;
   const int64_t ks = o.ks, n = o.n, max_items = o.max_items;
-  const int64_t max_parts = 2 * ceil_div(n, DEPOSIT_ITEM_MAX);
+  const int64_t max_parts = 2 * ceil_div(n, (int64_t)o.item_max);
   double *partials = static_cast<double *>(p.partials.ensure((size_t)max_parts * tg.cells * 8 + 64));
 
   DepParams P;

@@ -99,6 +99,7 @@ struct TileOrder {
   // smallest nonzero |mass| (0xffffffff when every mass is 0)
   const uint32_t *mrange = nullptr;
   int64_t ks = 0, n = 0, max_items = 0;
+  uint32_t item_max = 0;  // records per work item at most (deposit_item_max)
 };
 
 // Tile order kept from the last deposit made with retain = true, for the

@@ -782,9 +782,10 @@ __global__ void k_tile_start_1pass(const uint32_t *__restrict__ t1scan, int64_t 
   tile_start[k] = (k == ks) ? (uint32_t)n : t1scan[(int64_t)k * g1];
 }
 
-// Work items per tile.  store mode: every real tile gets at least one item
-// (so untouched tiles are written as zeros); add mode: empty tiles are skipped.
-__global__ void k_items(const uint32_t *__restrict__ tile_start, int64_t ks, int64_t kreal, int store,
+// Work items per tile, at most imax records each.  store mode: every real
+// tile gets at least one item (so untouched tiles are written as zeros); add
+// mode: empty tiles are skipped.
+__global__ void k_items(const uint32_t *__restrict__ tile_start, int64_t ks, int64_t kreal, int store, uint32_t imax,
                         uint32_t *__restrict__ mcnt, uint32_t *__restrict__ pcnt, uint32_t *__restrict__ nheavy) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k > ks) return;
@@ -795,7 +796,7 @@ __global__ void k_items(const uint32_t *__restrict__ tile_start, int64_t ks, int
   }
   const uint32_t c = tile_start[k + 1] - tile_start[k];
   uint32_t m = 0;
-  if (k < kreal) m = c == 0 ? (store ? 1u : 0u) : (c + DEPOSIT_ITEM_MAX - 1) / DEPOSIT_ITEM_MAX;
+  if (k < kreal) m = c == 0 ? (store ? 1u : 0u) : (c + imax - 1) / imax;
   mcnt[k] = m;
   pcnt[k] = m > 1 ? m : 0;
   if (k == 0) *nheavy = 0;
@@ -1072,14 +1073,15 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
   uint32_t *mcnt = ib, *item_start = ib + (ks + 1), *pcnt = ib + 2 * (ks + 1), *part_start = ib + 3 * (ks + 1);
   uint32_t *hv = static_cast<uint32_t *>(p.heavy.ensure((size_t)(ks + 2) * 4));
   uint32_t *nheavy = hv, *heavy = hv + 1;
+  const uint32_t imax = deposit_item_max(n, tg.K);
   hipLaunchKernelGGL(k_items, dim3((unsigned)ceil_div(ks + 1, 256)), dim3(256), 0, s, tile_start, ks, tg.K, store,
-                     mcnt, pcnt, nheavy);
+                     imax, mcnt, pcnt, nheavy);
   GENPK_LAUNCH_CHECK();
   exclusive_scan_u32(mcnt, item_start, ks + 1, p.scan_tmp, s);
   exclusive_scan_u32(pcnt, part_start, ks + 1, p.scan_tmp, s);
   hipLaunchKernelGGL(k_heavy_list, dim3((unsigned)ceil_div(ks, 256)), dim3(256), 0, s, pcnt, ks, nheavy, heavy);
   GENPK_LAUNCH_CHECK();
-  const int64_t max_items = tg.K + ceil_div(n, DEPOSIT_ITEM_MAX);
+  const int64_t max_items = tg.K + ceil_div(n, (int64_t)imax);
   uint32_t *item_tile = static_cast<uint32_t *>(p.item_tile.ensure((size_t)max_items * 4));
   hipLaunchKernelGGL(k_item_tiles, dim3((unsigned)ceil_div(ks, 256)), dim3(256), 0, s, item_start, ks, item_tile);
   GENPK_LAUNCH_CHECK();
@@ -1101,9 +1103,16 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
   o.ks = ks;
   o.n = n;
   o.max_items = max_items;
+  o.item_max = imax;
   return o;
 }
 }  // namespace
+
+uint32_t deposit_item_max(int64_t n, int64_t tiles) {
+  uint32_t m = DEPOSIT_ITEM_MAX;
+  while (m > DEPOSIT_ITEM_MIN && tiles + n / m < 4 * (int64_t)num_cus()) m /= 2;
+  return m;
+}
 
 KeyGeom key_geom(const Plan &p, double boxsize) {
   KeyGeom g;

@@ -96,8 +96,14 @@ __device__ __forceinline__ bool item_range(const uint32_t *__restrict__ item_til
   return true;
 }
 
+// Records per work item at most for n particles over `tiles` tiles:
+// DEPOSIT_ITEM_MAX, halved (down to DEPOSIT_ITEM_MIN) while the items would
+// number fewer than four per CU, so that a few heavy tiles still spread
+// over the CUs (their items' partial tiles are summed afterwards).
+uint32_t deposit_item_max(int64_t n, int64_t tiles);
+
 // Sort the n particles of one call (n < 2^31) by tile into plan scratch and
-// build the work items (at most DEPOSIT_ITEM_MAX records each).  Records are
+// build the work items (at most deposit_item_max records each).  Records are
 // x, y, z, then the mass when masses != nullptr.  with_perm: also record
 // where every particle went (perm1: input -> pass-1 position; perm2, two
 // passes only: pass-1 -> final position), for taking results back to the
