@@ -1109,8 +1109,9 @@ TileOrder sort_impl(Plan &p, const KeyGeom &kg, int64_t n, const float *pos, con
 }  // namespace
 
 uint32_t deposit_item_max(int64_t n, int64_t tiles) {
+  const int64_t cus = num_cus();
   uint32_t m = DEPOSIT_ITEM_MAX;
-  while (m > DEPOSIT_ITEM_MIN && tiles + n / m < 4 * (int64_t)num_cus()) m /= 2;
+  while (m > DEPOSIT_ITEM_MIN && (tiles + n / m < 4 * cus || (int64_t)m * 2 * cus > n)) m /= 2;
   return m;
 }
 

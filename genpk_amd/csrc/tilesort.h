@@ -98,7 +98,8 @@ __device__ __forceinline__ bool item_range(const uint32_t *__restrict__ item_til
 
 // Records per work item at most for n particles over `tiles` tiles:
 // DEPOSIT_ITEM_MAX, halved (down to DEPOSIT_ITEM_MIN) while the items would
-// number fewer than four per CU, so that a few heavy tiles still spread
+// number fewer than four per CU or one full item would hold more than half
+// of a CU's share of the particles, so that a few heavy tiles still spread
 // over the CUs (their items' partial tiles are summed afterwards).
 uint32_t deposit_item_max(int64_t n, int64_t tiles);
 
