@@ -118,14 +118,24 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
       }
     }
   };
-  // Persistent over the items (item, item + gridDim.x, ...): the next item's
-  // tile lookup and first records load while this item's tile is flushed.
+  auto take_batch = [&]() {
+#pragma unroll
+    for (int u = 0; u < UNR; u++)
+#pragma unroll
+      for (int q = 0; q < RW; q++) r[u][q] = nx[u][q];
+  };
+  // Persistent over the items (item, item + gridDim.x, ...).  The next
+  // item's first records load during this item's last batch and are in
+  // registers before the flush: the flush's stores then drain while the next
+  // item accumulates (a wait for loads issued after them would wait for the
+  // stores too, the counter being in order).
   uint32_t item = blockIdx.x;
   ItemRange ir;
   bool have = item_range(item_tile, item_start, tile_start, P.ks, P.max_items, item, ir);
   if (have) load_batch(ir.pb, ir.pe);
   // the tile starts cleared; each flush clears the cells it reads
   for (int64_t c = threadIdx.x; c < P.cells; c += DEP_THREADS) acc[c] = 0;
+  take_batch();
   __syncthreads();
   while (have) {
     const int64_t k = ir.k;
@@ -136,13 +146,20 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
     const int tx = (int)(k / ((int64_t)ntz * nty));
     const int ox = tx * TX, oy = ty * TY, oz = tz * TZ;
     const int wx = min(TX, P.kg.d - ox), wy = min(TY, P.kg.d - oy), wz = min(TZ, P.kg.d - oz);  // the tile's own cells
+    // the next item (uniform over the workgroup)
+    const uint32_t nitem = item + gridDim.x;
+    ItemRange nir;
+    const bool next = item_range(item_tile, item_start, tile_start, P.ks, P.max_items, nitem, nir);
+    bool pre = false;  // the next item's first records are loading
     for (uint32_t i0 = pb; i0 < pe; i0 += BATCH) {
-#pragma unroll
-      for (int u = 0; u < UNR; u++)
-#pragma unroll
-        for (int q = 0; q < RW; q++) r[u][q] = nx[u][q];
-      // loads of the next batch are issued before this batch is accumulated
-      if (i0 + BATCH < pe) load_batch(i0 + BATCH, pe);
+      // loads of the next batch (this item's, or the next item's first) are
+      // issued before this batch (in r) is accumulated; one load site (two
+      // would merge their registers with copies that wait for the loads)
+      const bool more = i0 + BATCH < pe;
+      if (more || next) {
+        load_batch(more ? i0 + BATCH : nir.pb, more ? pe : nir.pe);
+        pre = !more;
+      }
 #pragma unroll
       for (int u = 0; u < UNR; u++) {
         const uint32_t i = i0 + u * DEP_THREADS + threadIdx.x;
@@ -199,13 +216,15 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
           }
         }
       }
+      take_batch();
+    }
+    // an item without records: the next item's first records load here
+    // (taken before the flush too, so that no load is left in flight across it)
+    if (next && !pre) {
+      load_batch(nir.pb, nir.pe);
+      take_batch();
     }
     __syncthreads();
-    // the next item (uniform over the workgroup); its first records load during the flush
-    item += gridDim.x;
-    ItemRange nir;
-    const bool next = item_range(item_tile, item_start, tile_start, P.ks, P.max_items, item, nir);
-    if (next) load_batch(nir.pb, nir.pe);
     if (m == 1) {
       // rows of TZ values (aligned), then the z-halo plane
       // (32-bit cell indices: a tile's accumulator holds fewer than 2^24 cells)
@@ -234,6 +253,7 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
       }
     }
     __syncthreads();  // the flush has read and cleared the tile before the next item adds to it
+    item = nitem;
     ir = nir;
     have = next;
   }
