@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_adjoint_x(typename Cx<T>::C *__
 // registers here: 5.2 -> 7.3 ms at 1024^3, against 4.3 ms without.)
 template <typename T, bool LT>
 __global__ void __launch_bounds__(ADJ_THREADS) k_adjoint_x_pp(typename Cx<T>::C *__restrict__ f, int64_t d,
-                                                             int64_t hz, int lgb, int nkb, RadixPlan rp,
+                                                             int64_t hz, int lgb, int nkb, FastDiv fnkb, RadixPlan rp,
                                                              const typename Cx<T>::C *__restrict__ tw, AdjWeight w) {
   using C = typename Cx<T>::C;
   constexpr int MAXE = AdjCfg<T>::MAXE;
@@ -173,8 +173,9 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_adjoint_x_pp(typename Cx<T>::C 
   // (a grid that is a multiple of the XCD count keeps vb on this workgroup's XCD)
   auto load = [&](int64_t vb) {
     const uint32_t blk = xcd_block((uint32_t)vb, (uint32_t)nblk);
-    const int64_t y = blk / nkb;
-    const int64_t kz0 = (int64_t)(blk - y * nkb) * B;
+    const uint32_t yb = fnkb.div(blk);  // a multiply-high, not a division loop
+    const int64_t y = yb;
+    const int64_t kz0 = (int64_t)(blk - yb * (uint32_t)nkb) * B;
     const int nb = (int)min((int64_t)B, hz - kz0);
     const C *src = f + y * hz + kz0;
 #pragma unroll
@@ -187,8 +188,9 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_adjoint_x_pp(typename Cx<T>::C 
   if ((int64_t)blockIdx.x < nblk) load(blockIdx.x);
   for (int64_t vb = blockIdx.x; vb < nblk; vb += gridDim.x) {
     const uint32_t blk = xcd_block((uint32_t)vb, (uint32_t)nblk);
-    const int64_t y = blk / nkb;
-    const int64_t kz0 = (int64_t)(blk - y * nkb) * B;
+    const uint32_t yb = fnkb.div(blk);
+    const int64_t y = yb;
+    const int64_t kz0 = (int64_t)(blk - yb * (uint32_t)nkb) * B;
     const int nb = (int)min((int64_t)B, hz - kz0);
     C *base = f + y * hz + kz0;
     // the previous block's stores have read their buffer before this barrier
@@ -442,10 +444,10 @@ void adjoint_x_impl(Plan &p, const AdjWeight &w) {
     const dim3 g((unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)std::max(per_cu, 1) * num_cus(), p.d * nkb)));
     if (lt)
       hipLaunchKernelGGL((k_adjoint_x_pp<T, true>), g, dim3(ADJ_THREADS), lds, p.stream, p.field.as<C>(), p.d, p.hz,
-                         ilog2(B), nkb, rp, tw, w);
+                         ilog2(B), nkb, FastDiv((uint32_t)nkb), rp, tw, w);
     else
       hipLaunchKernelGGL((k_adjoint_x_pp<T, false>), g, dim3(ADJ_THREADS), lds, p.stream, p.field.as<C>(), p.d, p.hz,
-                         ilog2(B), nkb, rp, tw, w);
+                         ilog2(B), nkb, FastDiv((uint32_t)nkb), rp, tw, w);
   } else {
     hipLaunchKernelGGL((k_adjoint_x<T>), grid, dim3(ADJ_THREADS), (size_t)(tot + p.d) * sizeof(C), p.stream,
                        p.field.as<C>(), p.d, p.hz, ilog2(B), nkb, rp, tw, w);

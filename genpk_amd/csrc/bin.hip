@@ -148,7 +148,7 @@ struct BinTrack {
 template <typename T, int NF, bool REG1, bool R2L = false, int CT = 0>
 __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__restrict__ fa,
                                                             typename Cx<T>::C *__restrict__ fb, BinGeom g, int B,
-                                                            int nkb, RadixPlan rp,
+                                                            int nkb, FastDiv fnkb, RadixPlan rp,
                                                             const typename Cx<T>::C *__restrict__ tw, int keep,
                                                             int pp, double *__restrict__ part) {
   using C = typename Cx<T>::C;
@@ -185,9 +185,10 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
   C va[MAXE], vb[NF == 2 ? MAXE : 1];
   auto load = [&](int64_t v) {
     // a grid that is a multiple of the XCD count keeps v on this workgroup's XCD
-    const int64_t blk = xcd_block((uint32_t)v, (uint32_t)nblk);
-    const int64_t y = blk / nkb;
-    const int64_t kz0 = (blk % nkb) * B;
+    const uint32_t blk = xcd_block((uint32_t)v, (uint32_t)nblk);
+    const uint32_t yb = fnkb.div(blk);  // 32-bit: a 64-bit division costs ~130 scalar instructions
+    const int64_t y = yb;
+    const int64_t kz0 = (int64_t)(blk - yb * (uint32_t)nkb) * B;
     const int nb = (int)min((int64_t)B, hz - kz0);
     const int64_t off = y * hz + kz0;
 #pragma unroll
@@ -203,9 +204,10 @@ __global__ void __launch_bounds__(BIN_THREADS) k_xpass_bin(typename Cx<T>::C *__
   // (their live range misses the FFT stages, where the registers peak)
   if ((int64_t)blockIdx.x < nblk) load(blockIdx.x);
   for (int64_t v = blockIdx.x; v < nblk; v += gridDim.x) {
-    const int64_t blk = xcd_block((uint32_t)v, (uint32_t)nblk);
-    const int64_t y = blk / nkb;
-    const int kb = (int)(blk % nkb);
+    const uint32_t blk = xcd_block((uint32_t)v, (uint32_t)nblk);
+    const uint32_t yb = fnkb.div(blk);
+    const int64_t y = yb;
+    const int kb = (int)(blk - yb * (uint32_t)nkb);
     const int64_t kz0 = (int64_t)kb * B;
     const int nb = (int)min((int64_t)B, hz - kz0);
     const int64_t off = y * hz + kz0;
@@ -593,7 +595,7 @@ static void launch_fused(Plan &a, Plan *b, const BinTables &bt, const RadixPlan 
   const size_t lds = fused_lds<T, NF>(a, B, bt.nrbins, pp);
   Stage st(a, "xpass_bin");
   hipLaunchKernelGGL((k_xpass_bin<T, NF, REG1, R2L, CT>), dim3(G), dim3(BIN_THREADS), lds, a.stream, a.field.as<C>(),
-                     b ? b->field.as<C>() : (C *)nullptr, g, B, nkb, rp, tw, keep ? 1 : 0, pp ? 1 : 0, part);
+                     b ? b->field.as<C>() : (C *)nullptr, g, B, nkb, FastDiv((uint32_t)nkb), rp, tw, keep ? 1 : 0, pp ? 1 : 0, part);
   GENPK_LAUNCH_CHECK();
 }
 
