@@ -269,6 +269,7 @@ struct GatherParams {
   int64_t rowlen;
   double mass;
   FastDiv fhz, fhy;
+  FastDiv fntz, fnty;  // tile index -> (tx, ty, tz)
 };
 
 // One workgroup per work item of the tile order: stage the halo'd tile of
@@ -296,9 +297,10 @@ __global__ void __launch_bounds__(GATHER_THREADS) k_gather_tiles(const T *__rest
   if (ir.pb >= ir.pe) return;
   const int64_t k = ir.k;
   const int ntz = P.kg.ntz, nty = P.kg.nty, d = P.kg.d;
-  const int tz = (int)(k % ntz);
-  const int ty = (int)((k / ntz) % nty);
-  const int tx = (int)(k / ((int64_t)ntz * nty));
+  const uint32_t kxy = P.fntz.div((uint32_t)k), kx = P.fnty.div(kxy);
+  const int tz = (int)((uint32_t)k - kxy * (uint32_t)ntz);
+  const int ty = (int)(kxy - kx * (uint32_t)nty);
+  const int tx = (int)kx;
   const int ox = tx << P.kg.sx, oy = ty << P.kg.sy, oz = tz << P.kg.sz;
   // four loads in flight per thread while staging, and four records while gathering
   constexpr int UNR = 4;
@@ -469,6 +471,8 @@ void gather_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass, fl
   P.mass = mass;
   P.fhz = FastDiv((uint32_t)tg.HZ);
   P.fhy = FastDiv((uint32_t)tg.HY);
+  P.fntz = FastDiv((uint32_t)tg.ntz);
+  P.fnty = FastDiv((uint32_t)tg.nty);
   const int64_t grid = ceil_div(o.max_items, 8 * GATHER_RUN) * 8 * GATHER_RUN;
   hipLaunchKernelGGL((k_gather_tiles<T, NM, GM>), dim3((unsigned)grid), dim3(GATHER_THREADS),
                      (size_t)tg.cells * sizeof(T), p.stream, p.field.as<const T>(), o.rec, o.tile_start, o.item_start,

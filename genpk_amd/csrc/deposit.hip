@@ -17,6 +17,7 @@
 // Per-particle index and weight arithmetic follows fieldize.cpp:63-92:
 // u = (double)pos * (d/box); i0 = floor(u) wrapped into [0,d), i1 = i0+1
 // wrapped; f = u - floor(u); corner c has bit0 = x, bit1 = y, bit2 = z.
+#include "fft_core.h"
 #include "tiles.h"
 #include "tilesort.h"
 
@@ -60,6 +61,7 @@ struct DepParams {
   const uint32_t *mrange;  // per-particle masses: the sort's |mass| range (float bits)
   uint32_t *nonfinite;     // set to 1 when a particle's position is not finite
   int store;           // 1: write tiles, 0: add to tiles
+  FastDiv fntz, fnty;  // tile index -> (tx, ty, tz) by multiply-high (tiles < 2^31)
 };
 
 // Records are RW floats: x, y, z, then the mass when NM.  Without masses the
@@ -141,9 +143,11 @@ __global__ void __launch_bounds__(DEP_THREADS) k_deposit(const float *__restrict
     const int64_t k = ir.k;
     const uint32_t m = ir.m, j = ir.j, pb = ir.pb, pe = ir.pe;
     const int ntz = P.kg.ntz, nty = P.kg.nty;
-    const int tz = (int)(k % ntz);
-    const int ty = (int)((k / ntz) % nty);
-    const int tx = (int)(k / ((int64_t)ntz * nty));
+    // 32-bit multiply-high, not 64-bit divisions (~130 scalar instructions each)
+    const uint32_t kxy = P.fntz.div((uint32_t)k), kx = P.fnty.div(kxy);
+    const int tz = (int)((uint32_t)k - kxy * (uint32_t)ntz);
+    const int ty = (int)(kxy - kx * (uint32_t)nty);
+    const int tx = (int)kx;
     const int ox = tx * TX, oy = ty * TY, oz = tz * TZ;
     const int wx = min(TX, P.kg.d - ox), wy = min(TY, P.kg.d - oy), wz = min(TZ, P.kg.d - oz);  // the tile's own cells
     // the next item (uniform over the workgroup)
@@ -475,6 +479,8 @@ void deposit_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass) {
   P.mrange = o.mrange;
   P.nonfinite = static_cast<uint32_t *>(p.nonfinite.ensure(64));
   P.store = store;
+  P.fntz = FastDiv((uint32_t)tg.ntz);
+  P.fnty = FastDiv((uint32_t)tg.nty);
   fill_zero(P.nonfinite, 4, s);
   const size_t lds = (size_t)tg.cells * 8;
   {

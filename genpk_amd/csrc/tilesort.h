@@ -90,7 +90,7 @@ __device__ __forceinline__ bool item_range(const uint32_t *__restrict__ item_til
   const uint32_t t0 = tile_start[k], t1 = tile_start[k + 1];
   r.m = i1 - i0;
   r.j = item - i0;
-  const uint32_t per = (t1 - t0 + r.m - 1) / r.m;
+  const uint32_t per = r.m == 1 ? t1 - t0 : (t1 - t0 + r.m - 1) / r.m;  // most tiles are one item
   r.pb = min(t1, t0 + r.j * per);
   r.pe = min(t1, r.pb + per);
   return true;
