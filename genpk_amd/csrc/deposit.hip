@@ -62,6 +62,7 @@ struct DepParams {
   uint32_t *nonfinite;     // set to 1 when a particle's position is not finite
   int store;           // 1: write tiles, 0: add to tiles
   FastDiv fntz, fnty;  // tile index -> (tx, ty, tz) by multiply-high (tiles < 2^31)
+  FastDiv fnblk;       // k_reduce_partials: unit -> (heavy tile, 256-cell block)
 };
 
 // Records are RW floats: x, y, z, then the mass when NM.  Without masses the
@@ -302,8 +303,10 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const uint32_t *__restr
   const int64_t nblk = ceil_div(cells, 256);
   const int64_t units = (int64_t)(*nheavy) * nblk;
   for (int64_t u = blockIdx.x; u < units; u += gridDim.x) {
-    const uint32_t k = heavy[u / nblk];
-    const int64_t c = (u % nblk) * 256 + threadIdx.x;
+    // units < 2^31 (heavy tiles < 2^22, blocks per tile < 2^8): 32-bit multiply-high
+    const uint32_t h = P.fnblk.div((uint32_t)u);
+    const uint32_t k = heavy[h];
+    const int64_t c = (int64_t)((uint32_t)u - h * (uint32_t)nblk) * 256 + threadIdx.x;
     if (c >= cells) continue;
     const uint32_t m = item_start[k + 1] - item_start[k];
     const int64_t at = (int64_t)part_start[k] * cells + c;
@@ -481,6 +484,7 @@ void deposit_impl(Plan &p, const KeyGeom &kg, const TileOrder &o, double mass) {
   P.store = store;
   P.fntz = FastDiv((uint32_t)tg.ntz);
   P.fnty = FastDiv((uint32_t)tg.nty);
+  P.fnblk = FastDiv((uint32_t)ceil_div(tg.cells, 256));
   fill_zero(P.nonfinite, 4, s);
   const size_t lds = (size_t)tg.cells * 8;
   {
