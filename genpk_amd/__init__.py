@@ -336,7 +336,9 @@ def fft_r2c_3d_inplace(field):
 
 def radix_plan(n, fft_variant=0):
     """Radices, in stage order, of the library's length-n transform under an
-    FFT variant (host only)."""
+    FFT variant (host only); variant -1: the radix-8-first, radix-2-last
+    ordering of an even number of stages that the FP32 fused x pass takes
+    where the plan's own ordering is not of that form."""
     buf = (ctypes.c_int * 32)()
     cnt = ctypes.c_int()
     check(lib().genpk_radix_plan(int(n), int(fft_variant), buf, 32, ctypes.byref(cnt)))

@@ -227,9 +227,11 @@ int genpk_plan_set_options(genpk_plan *plan, int tile_x, int tile_y, int tile_z,
 int genpk_radix_plan(int64_t n, int fft_variant, int *radices, int max, int *count) {
   return guarded([&] {
     GENPK_REQUIRE(radices && count && max >= 0, "null argument");
-    GENPK_REQUIRE(fft_variant >= 0 && fft_variant < FFT_VARIANTS, "unknown FFT radix variant");
+    GENPK_REQUIRE(fft_variant >= -1 && fft_variant < FFT_VARIANTS, "unknown FFT radix variant");
     GENPK_REQUIRE(fft_length_ok(n), "transform length must be 2^a 3^b 5^c, at most 2^20");
-    const RadixPlan rp = make_radix_plan((int)n, fft_variant);
+    const RadixPlan rp = fft_variant < 0 ? radix_plan_8_to_2((int)n) : make_radix_plan((int)n, fft_variant);
+    GENPK_REQUIRE(rp.nst >= 0, "no ordering with a first radix 8, a last radix 2 and an even number of stages "
+                               "for this length");
     GENPK_REQUIRE(rp.nst <= max, "radices buffer too short");
     for (int s = 0; s < rp.nst; s++) radices[s] = rp.st[s].R;
     *count = rp.nst;

@@ -69,7 +69,10 @@ int genpk_plan_set_options(genpk_plan *plan, int tile_x, int tile_y, int tile_z,
 int genpk_plan_get_options(const genpk_plan *plan, int *tile_x, int *tile_y, int *tile_z, int *fft_variant);
 /* The radices, in stage order, of a length-n transform under fft_variant
  * (host only; the y and x passes use n = dims, the z passes n = dims/2,
- * with ordering 0 under variant 3). */
+ * with ordering 0 under variant 3).  fft_variant -1: the ordering with a
+ * first radix 8, a last radix 2 and an even number of stages that the FP32
+ * fused x pass + bin takes where the plan's own ordering is not of that
+ * form (an error for lengths without one, such as 32 and 48). */
 int genpk_radix_plan(int64_t n, int fft_variant, int *radices, int max, int *count);
 
 /* Empty the density (the reference zeroes the grid, gen-pk.cpp:208). */

@@ -105,3 +105,34 @@ def test_radix_orderings():
                 assert r[0] == 4
     with pytest.raises(genpk_amd.GenPKError):
         genpk_amd.radix_plan(14, 0)
+
+
+def test_fused_x_orderings():
+    """The orderings the FP32 fused x pass takes (variant -1): a first radix 8
+    (the stage run from registers), a last radix 2 (the stage run in the
+    binning) and an even number of stages (the result back in the first
+    buffer), the fewest stages for powers of two."""
+    import genpk_amd
+    known = {16: [8, 2], 64: [8, 2, 2, 2], 96: [8, 2, 3, 2], 128: [8, 4, 2, 2], 256: [8, 4, 4, 2],
+             384: [8, 8, 3, 2], 512: [8, 8, 4, 2], 1024: [8, 8, 8, 2], 2048: [8, 4, 4, 4, 2, 2]}
+    for n, r in known.items():
+        assert genpk_amd.radix_plan(n, -1) == r
+    made = 0
+    for n in range(16, 8193, 16):
+        try:
+            r = genpk_amd.radix_plan(n, -1)
+        except genpk_amd.GenPKError:
+            continue
+        made += 1
+        assert int(np.prod(r)) == n and set(r) <= {2, 3, 4, 5, 8}
+        assert r[0] == 8 and r[-1] == 2 and len(r) % 2 == 0
+        if n & (n - 1) == 0:
+            # the fewest stages of radix <= 8 between a first 8 and a last 2, in an even count
+            assert len(r) == 2 + 2 * -(-(n.bit_length() - 5) // 6)
+    assert made >= 50  # 53 of the 512 multiples of 16 up to 8192
+    # no such ordering: 32 = 8 4, 48 = 16 * 3, 80 = 16 * 5, 480 = 16 * 2 * 3 * 5
+    # (an odd number of middle radices with no 8 or 4 to split), 40 (not a
+    # multiple of 16), 112 (a factor 7)
+    for n in (32, 48, 80, 480, 40, 112):
+        with pytest.raises(genpk_amd.GenPKError):
+            genpk_amd.radix_plan(n, -1)
