@@ -40,9 +40,10 @@ template <> struct FftCfg<double> { static constexpr int ELEMS = 4096; static co
 // FFT_THREADS): each thread loads the eight inputs of up to two butterflies
 // of the first stage straight from the tile store and runs that stage from
 // registers.  CT = 512: the plan is M = 512 = 8 * 8 * 8 (d = 1024), run with
-// compile-time stages (fft_stage_ct); 0: any plan.  STD (with BFLY and CT):
+// compile-time stages (fft_stage_ct); 0: any plan.  STD (with CT = 512):
 // the default 16 x 32 x 32 tiles, whose store offsets (below 2^31 at d =
-// 1024) are computed in 32 bits from compile-time strides.
+// 1024) are computed in 32 bits from compile-time strides, in the butterfly
+// loader (FP32) or the element-wise one (FP64).
 template <typename T, bool FROM_TILES, bool BFLY = false, int CT = 0, bool STD = false>
 __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(1, 2))) k_r2c_z(T *__restrict__ field, const T *__restrict__ tiles,
                                                       FoldGeom fg, FastDiv fdd, int64_t rowlen, int64_t nrows, int L,
@@ -71,7 +72,7 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
       if (b < L) {
         C w[8];
         if (STD && b < nl) {
-          static_assert(!STD || (BFLY && CT == 512), "standard tiles with the d = 1024 butterfly loader");
+          static_assert(!STD || CT == 512, "standard tiles with the d = 1024 loaders");
           // d = 1024, tiles 16 x 32 x 32 (17 x 33 halo'd rows of 32 per tile):
           // z = 2 (j + 64 r) lies in tile column tz = 4 r + j / 16 at lz = 2 (j % 16)
           constexpr uint32_t CM = 17 * 33 * 32, CZ = 17 * 33;
@@ -136,7 +137,26 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
   } else {
   C v[MAXE];
   T zh[FROM_TILES ? MAXE : 1];
-  if (FROM_TILES && M == FFT_THREADS) {
+  if (FROM_TILES && M == FFT_THREADS && STD) {
+    // d = 1024 with the default tiles (the FP64 plan): as below, with 32-bit
+    // store offsets from compile-time strides (the store holds fewer than
+    // 2^31 values; every tile is full, so a z halo sits in the z-halo plane)
+    constexpr uint32_t CM = 17 * 33 * 32, CZ = 17 * 33;
+    const uint32_t z = 2u * threadIdx.x, tz = z >> 5, lz = z & 31u, tzp = (tz + 31u) & 31u;
+#pragma unroll
+    for (int q = 0; q < MAXE; q++) {
+      v[q] = C{0, 0};
+      zh[FROM_TILES ? q : 0] = (T)0;
+      if (q < L && q < nl) {
+        const uint32_t row = (uint32_t)(row0 + q);
+        const uint32_t x = row >> 10, y = row & 1023u;
+        const uint32_t txy = (x >> 4) * 32u + (y >> 5), rr = (x & 15u) * 33u + (y & 31u);
+        const uint32_t at = (txy * 32u + tz) * CM + rr * 32u + lz;
+        v[q] = C{tiles[at], tiles[at + 1]};
+        if (lz == 0) zh[FROM_TILES ? q : 0] = tiles[(uint32_t)fg.zoff + (txy * 32u + tzp) * CZ + rr];
+      }
+    }
+  } else if (FROM_TILES && M == FFT_THREADS) {
     // element q of this thread is row row0 + q, reals 2m and 2m + 1 with m =
     // threadIdx.x: the row's part of the store offset is the same for the
     // whole workgroup (scalar), the z part is the lane's own
@@ -548,6 +568,11 @@ static void r2c_z_impl(Plan &p, bool from_tiles) {
   } else if (from_tiles && bfly) {
     Stage st(p, "fold_fft_z");
     hipLaunchKernelGGL((k_r2c_z<T, true, true>), grid, dim3(FFT_THREADS), lds, p.stream, p.field.as<T>(),
+                       p.tiles.as<const T>(), fg, FastDiv((uint32_t)p.d), p.rowlen, nrows, L, FastDiv((uint32_t)M),
+                       FastDiv((uint32_t)(M + 1)), rp, twM, twd);
+  } else if (from_tiles && ct512 && std_tiles) {
+    Stage st(p, "fold_fft_z");
+    hipLaunchKernelGGL((k_r2c_z<T, true, false, 512, true>), grid, dim3(FFT_THREADS), lds, p.stream, p.field.as<T>(),
                        p.tiles.as<const T>(), fg, FastDiv((uint32_t)p.d), p.rowlen, nrows, L, FastDiv((uint32_t)M),
                        FastDiv((uint32_t)(M + 1)), rp, twM, twd);
   } else if (from_tiles && ct512) {
